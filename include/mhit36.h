@@ -1,0 +1,195 @@
+/*
+ * mhit36.h -- C-ABI of the MI355X-native MHIT36 single-GPU time-stepper.
+ *
+ * The reference (aroccon/MHIT36_mixedp) has no operator API: its whole
+ * timestep is inline in `program main` (multi/main.f90:456-1042) and talks to
+ * cuFFT, cuDecomp and MPI directly.  This header is the drop-in boundary that
+ * replaces those call sites for one GPU.  A Fortran driver binds it through
+ * iso_c_binding (driver/mhit_c.f90); Python binds it through ctypes
+ * (mhit36_mixedp_amd/api.py).
+ *
+ * Conventions
+ *  - Every call returns an int status, MHIT_OK (0) on success.  This mirrors
+ *    CHECK_CUDECOMP_EXIT (multi/main.f90:1); unlike the reference, FFT/HIP
+ *    errors are never ignored (cf. main.f90:190-205, 854-913).  The text of
+ *    the last error is returned by mhit_last_error().
+ *  - Fields are interior n^3 FP64 arrays in Fortran order (x fastest), i.e.
+ *    C order [z][y][x], exactly the layout of the raw .dat files written by
+ *    writefield (multi/readwrite.f90:39-92).  Host buffers are caller-owned;
+ *    uploads and downloads copy.
+ *  - One host thread per context, one HIP stream per context.  Calls are
+ *    blocking at return, as the reference runs everything synchronously on
+ *    the default stream.
+ */
+#ifndef MHIT36_H
+#define MHIT36_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MHIT_OK              0
+#define MHIT_ERR_ARG         1  /* bad argument / unsupported size          */
+#define MHIT_ERR_HIP         2  /* HIP runtime error                        */
+#define MHIT_ERR_CFL         3  /* cou > 7: the reference stops (main.f90:1006) */
+#define MHIT_ERR_NOMEM       4  /* device allocation failed                 */
+#define MHIT_ERR_STATE       5  /* call not valid in the current state      */
+#define MHIT_ERR_NOGPU       6  /* no HIP device visible                    */
+
+/* Poisson precision: the "mixedp" branch solves in FP32 (README.md:7-8,
+ * main.f90:283-284); the FP64 form is the same algorithm in double.      */
+#define MHIT_POISSON_FP64    0
+#define MHIT_POISSON_FP32    1
+
+/* Wavenumber used by the spectral divide (main.f90:889).                  */
+#define MHIT_K2_CONTINUOUS   0  /* kx^2+ky^2+kz^2, the reference           */
+#define MHIT_K2_MODIFIED     1  /* sum (2 sin(k dx/2)/dx)^2, opt-in: exact discrete projection */
+
+/* Tracer interpolation (particles.f90:78-122).                            */
+#define MHIT_INTERP_REFERENCE 0 /* 1-D linear per component, the reference */
+#define MHIT_INTERP_TRILINEAR 1 /* trilinear at the staggered positions, opt-in */
+
+/* Field numbers, as writefield/readfield (multi/readwrite.f90:21-25).      */
+#define MHIT_FIELD_U   1
+#define MHIT_FIELD_V   2
+#define MHIT_FIELD_W   3
+#define MHIT_FIELD_P   4
+#define MHIT_FIELD_PHI 5
+
+/*
+ * Run parameters.  The first block is input.inp, in its order
+ * (multi/input.inp:1-18, multi/readinput.f90:14-35).  The second block is
+ * what the reference fixes at compile time (module.f90:2 nx; main.f90:55-57
+ * phiflag/partflag) or this build adds as options.
+ */
+typedef struct mhit_params {
+    /* input.inp */
+    int32_t restart;      /* 0 fresh start, 1 restart                    */
+    int32_t tstart;       /* first iteration                             */
+    int32_t tfin;         /* last iteration                              */
+    int32_t dump;         /* output frequency                            */
+    int32_t inflow;       /* 0 Taylor-Green, 1 read ./input/{u,v,w}.dat  */
+    int32_t inphi;        /* 0 centred drop, 1 read ./input/phi.dat      */
+    double  dt;
+    double  mu;           /* dynamic viscosity                           */
+    double  rho;
+    double  f1, f2, f3;   /* ABC forcing amplitudes A, B, C              */
+    double  k0;           /* forced wavenumber                           */
+    double  radius;       /* initial drop radius                         */
+    double  sigma;        /* surface tension                             */
+    double  epsr;         /* eps/dx                                      */
+    int64_t npart;        /* number of tracers                           */
+    int32_t inpart;       /* tracer initial condition                    */
+    /* compile-time switches of the reference, runtime here               */
+    int32_t nx;           /* grid points per direction (ny = nz = nx)    */
+    int32_t phiflag;      /* 1: ACDI phase field + CSF                   */
+    int32_t partflag;     /* 1: Lagrangian tracers                       */
+    int32_t poisson_precision; /* MHIT_POISSON_FP32 (default) / _FP64    */
+    int32_t k2_mode;      /* MHIT_K2_CONTINUOUS (default) / _MODIFIED    */
+    int32_t interp_mode;  /* MHIT_INTERP_REFERENCE (default) / _TRILINEAR */
+    int32_t cfl_abort;    /* 1 (default): stop when cou > 7              */
+} mhit_params;
+
+/* Per-step scalars the reference keeps in module `velocity`
+ * (module.f90:59-61) plus diagnostics it never computed (added here).     */
+typedef struct mhit_diag {
+    int64_t step;         /* completed timesteps since create/upload     */
+    double  alpha, beta;  /* AB2 coefficients for the next step          */
+    double  gumax;        /* signed max over u,v,w (main.f90:998-1002)   */
+    double  cou;          /* gumax*dt/dx (main.f90:1003)                 */
+    double  umean, vmean, wmean; /* means removed by the last correction */
+    double  ke;           /* 0.5 <u^2+v^2+w^2> over stored face values   */
+    double  dissipation;  /* 2 nu <S_ij S_ij>, central differences       */
+    double  maxdiv;       /* max |sum_d (u_d(+1)-u_d)/dx|                 */
+    double  phi_mass;     /* sum phi dx^3                                */
+    double  phi_min, phi_max;
+    int64_t nan_count;    /* non-finite values seen in u,v,w,phi         */
+} mhit_diag;
+
+typedef struct mhit_ctx mhit_ctx;
+
+/* Reference defaults: multi/input.inp values, nx = 256 (module.f90:2),
+ * FP32 Poisson, continuous k^2, reference interpolation.                  */
+int mhit_params_default(mhit_params* p);
+
+/* Allocate device state and FFT plans for params p (main.f90:154-310).
+ * Supported nx: 2^a 3^b with 8 <= nx <= 512.  AB2 state starts as the
+ * reference's (alpha=1, beta=0, gumax=1; main.f90:440-444).               */
+int mhit_create(const mhit_params* p, mhit_ctx** out);
+int mhit_destroy(mhit_ctx* ctx);
+
+/* Copy an interior n^3 FP64 field in/out (fieldn 1..5, see MHIT_FIELD_*).
+ * Stand-in for readfield/readfield_restart/writefield
+ * (multi/readwrite.f90:1-318).  Downloading U/V/W returns the velocity
+ * after the mean removal of main.f90:977-981.  P is the pressure of the
+ * last projection promoted to FP64.                                        */
+int mhit_upload_field(mhit_ctx* ctx, int fieldn, const double* host);
+int mhit_download_field(mhit_ctx* ctx, int fieldn, double* host);
+
+/* Tracers (particles.f90:11-22): positions in [0, 2pi).  Upload resets the
+ * previous-step velocity to zero (the first step is Euler).  Download takes
+ * NULL for any column not wanted; `tag` returns the particle tag
+ * (part(:,Itag)) so that callers can match particles after reordering.    */
+int mhit_upload_tracers(mhit_ctx* ctx, int64_t n, const double* x,
+                        const double* y, const double* z);
+int mhit_download_tracers(mhit_ctx* ctx, double* tag, double* x, double* y,
+                          double* z, double* up, double* vp, double* wp);
+int64_t mhit_tracer_count(const mhit_ctx* ctx);
+
+/* Advance nsteps timesteps (the loop body main.f90:456-1007).  Returns
+ * MHIT_ERR_CFL if cou > 7 after some step (when cfl_abort = 1); the state
+ * is then the one after that step and further calls return MHIT_ERR_CFL. */
+int mhit_step(mhit_ctx* ctx, int nsteps);
+
+/* Scalars of the last step plus diagnostics computed on the current state. */
+int mhit_diag_compute(mhit_ctx* ctx, mhit_diag* out);
+
+/* Set the AB2 / gamma state, e.g. to resume a run (main.f90:440-444).     */
+int mhit_set_ab2_state(mhit_ctx* ctx, double alpha, double beta, double gumax);
+
+/* Block until all work queued on the context's stream is done.            */
+int mhit_sync(mhit_ctx* ctx);
+
+/* Text of the last error on this thread ("" if none).                    */
+const char* mhit_last_error(void);
+
+/* ---- initial conditions (computed on the device) -------------------- */
+/* Taylor-Green velocity, main.f90:324-337 (in the time-stepper's frame).  */
+int mhit_init_taylor_green(mhit_ctx* ctx);
+/* One drop of radius p.radius at the box centre, main.f90:370-382.        */
+int mhit_init_drop(mhit_ctx* ctx);
+/* Seeded random-phase velocity, |A(k)| ~ k^-2 on 0 < |k| <= kmax, exactly
+ * divergence-free for the staggered divergence, scaled to u_rms.          */
+int mhit_init_random_velocity(mhit_ctx* ctx, uint64_t seed, double kmax,
+                              double urms);
+/* ndrops seeded non-overlapping drops of the given radius (tanh profile of
+ * main.f90:378, periodic minimum-image distance, max over drops).         */
+int mhit_init_drops(mhit_ctx* ctx, int32_t ndrops, double radius,
+                    uint64_t seed);
+/* Drop centres used by mhit_init_drops (host only; xyz interleaved).      */
+int mhit_drop_centers(int32_t ndrops, double radius, double min_gap,
+                      uint64_t seed, double* xyz);
+/* n tracers uniform in the box from a seeded counter-based generator.     */
+int mhit_init_tracers_uniform(mhit_ctx* ctx, int64_t n, uint64_t seed);
+
+/* ---- building blocks, exposed for parity tests ---------------------- */
+/* Solve lap(p) = rhs with the context's Poisson path: R2C FFT, divide by
+ * -k^2 (zero mode 0), C2R (main.f90:853-912).  Host n^3 FP64 in and out;
+ * in FP32 mode rhs is rounded to FP32 first, as rhspp (main.f90:844).     */
+int mhit_poisson_solve(mhit_ctx* ctx, const double* rhs, double* p);
+/* Forward unnormalised 3-D R2C FFT (e^{-i}), output (n/2+1) x n x n
+ * complex as interleaved re,im in C order [kz][ky][kx].                   */
+int mhit_fft_r2c(mhit_ctx* ctx, const double* in, double* out);
+/* Inverse unnormalised 3-D C2R FFT (e^{+i}) of the same layout.           */
+int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
+
+/* Library version string.                                                 */
+const char* mhit_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MHIT36_H */

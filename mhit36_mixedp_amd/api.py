@@ -1,0 +1,177 @@
+"""Python host mirror of the C-ABI (include/mhit36.h) over ctypes.
+
+``Solver`` plays the role of the reference's ``program main``
+(multi/main.f90): it owns one device context, takes input.inp parameters,
+loads initial fields, advances the timestep and reads fields back.  Every
+call goes to the HIP library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import Diag, Params, default_params
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MhitError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        lib = _abi.load_library()
+        msg = lib.mhit_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+class CFLStop(MhitError):
+    """cou > 7: the reference stops the run (multi/main.f90:1006)."""
+
+
+def _check(rc: int, where: str):
+    if rc == _abi.OK:
+        return
+    if rc == _abi.ERR_CFL:
+        raise CFLStop(rc, where)
+    raise MhitError(rc, where)
+
+
+def _ptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def version() -> str:
+    return _abi.load_library().mhit_version().decode()
+
+
+def drop_centers(ndrops: int, radius: float, min_gap: float, seed: int) -> np.ndarray:
+    """Centres placed by ``Solver.init_drops`` (host-side generator)."""
+    out = np.zeros((max(ndrops, 1), 3))
+    _check(_abi.load_library().mhit_drop_centers(ndrops, radius, min_gap, seed, _ptr(out)),
+           "mhit_drop_centers")
+    return out[:ndrops]
+
+
+class Solver:
+    """One MHIT36 box on one GPU."""
+
+    def __init__(self, params: Params | None = None, **overrides):
+        self.lib = _abi.load_library()
+        self.params = params if params is not None else default_params(**overrides)
+        if params is not None and overrides:
+            for k, v in overrides.items():
+                setattr(self.params, k, v)
+        self.n = int(self.params.nx)
+        h = ctypes.c_void_p()
+        _check(self.lib.mhit_create(ctypes.byref(self.params), ctypes.byref(h)), "mhit_create")
+        self._h = h
+
+    # context management ---------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            self.lib.mhit_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # fields ---------------------------------------------------------------
+    def upload_field(self, fieldn: int, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != (self.n,) * 3:
+            raise ValueError(f"field must be {(self.n,) * 3} (z, y, x), got {a.shape}")
+        _check(self.lib.mhit_upload_field(self._h, fieldn, _ptr(a)), "mhit_upload_field")
+
+    def download_field(self, fieldn: int) -> np.ndarray:
+        out = np.empty((self.n,) * 3, dtype=np.float64)
+        _check(self.lib.mhit_download_field(self._h, fieldn, _ptr(out)), "mhit_download_field")
+        return out
+
+    def set_velocity(self, u, v, w):
+        for f, a in zip((_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W), (u, v, w)):
+            self.upload_field(f, a)
+
+    def velocity(self):
+        return tuple(self.download_field(f) for f in (_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W))
+
+    # tracers ----------------------------------------------------------------
+    def upload_tracers(self, x, y, z):
+        x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+        _check(self.lib.mhit_upload_tracers(self._h, x.size, _ptr(x), _ptr(y), _ptr(z)),
+               "mhit_upload_tracers")
+
+    def download_tracers(self) -> dict:
+        n = self.lib.mhit_tracer_count(self._h)
+        cols = {k: np.empty(n) for k in ("tag", "x", "y", "z", "up", "vp", "wp")}
+        _check(self.lib.mhit_download_tracers(self._h, *[_ptr(cols[k]) for k in
+                                                         ("tag", "x", "y", "z", "up", "vp", "wp")]),
+               "mhit_download_tracers")
+        return cols
+
+    # time stepping ----------------------------------------------------------
+    def step(self, nsteps: int = 1):
+        _check(self.lib.mhit_step(self._h, int(nsteps)), "mhit_step")
+
+    def set_ab2_state(self, alpha: float, beta: float, gumax: float):
+        _check(self.lib.mhit_set_ab2_state(self._h, alpha, beta, gumax), "mhit_set_ab2_state")
+
+    def sync(self):
+        _check(self.lib.mhit_sync(self._h), "mhit_sync")
+
+    def diag(self) -> dict:
+        d = Diag()
+        _check(self.lib.mhit_diag_compute(self._h, ctypes.byref(d)), "mhit_diag_compute")
+        return d.as_dict()
+
+    # initial conditions -------------------------------------------------------
+    def init_taylor_green(self):
+        _check(self.lib.mhit_init_taylor_green(self._h), "mhit_init_taylor_green")
+
+    def init_drop(self):
+        _check(self.lib.mhit_init_drop(self._h), "mhit_init_drop")
+
+    def init_random_velocity(self, seed: int, kmax: float = 8.0, urms: float = 1.0):
+        _check(self.lib.mhit_init_random_velocity(self._h, seed, kmax, urms),
+               "mhit_init_random_velocity")
+
+    def init_drops(self, ndrops: int, radius: float, seed: int):
+        _check(self.lib.mhit_init_drops(self._h, ndrops, radius, seed), "mhit_init_drops")
+
+    def init_tracers_uniform(self, n: int, seed: int):
+        _check(self.lib.mhit_init_tracers_uniform(self._h, n, seed), "mhit_init_tracers_uniform")
+
+    # building blocks ------------------------------------------------------------
+    def poisson_solve(self, rhs: np.ndarray) -> np.ndarray:
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        out = np.empty_like(rhs)
+        _check(self.lib.mhit_poisson_solve(self._h, _ptr(rhs), _ptr(out)), "mhit_poisson_solve")
+        return out
+
+    def fft_r2c(self, x: np.ndarray) -> np.ndarray:
+        """Unnormalised forward 3-D R2C FFT; returns complex [kz, ky, kx<=n/2]."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n = self.n
+        out = np.empty((n, n, n // 2 + 1, 2))
+        _check(self.lib.mhit_fft_r2c(self._h, _ptr(x), _ptr(out)), "mhit_fft_r2c")
+        return out[..., 0] + 1j * out[..., 1]
+
+    def fft_c2r(self, X: np.ndarray) -> np.ndarray:
+        """Unnormalised inverse 3-D C2R FFT of a [kz, ky, kx<=n/2] spectrum."""
+        n = self.n
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        assert X.shape == (n, n, n // 2 + 1)
+        buf = np.ascontiguousarray(np.stack([X.real, X.imag], axis=-1))
+        out = np.empty((n, n, n))
+        _check(self.lib.mhit_fft_c2r(self._h, _ptr(buf), _ptr(out)), "mhit_fft_c2r")
+        return out

@@ -1,0 +1,640 @@
+// api.hip -- the C-ABI of include/mhit36.h: context lifetime, transfers,
+// the timestep driver and the initial-condition generators.
+#include "fft.hpp"
+#include "internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace mhit {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+static uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+static double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
+template <typename P>
+static int dmalloc(P** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return MHIT_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(P));
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc of ") + std::to_string(count * sizeof(P)) + " bytes: " +
+                  hipGetErrorString(e));
+        *p = nullptr;
+        return MHIT_ERR_NOMEM;
+    }
+    return MHIT_OK;
+}
+
+static void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+static int check_params(const mhit_params* p) {
+    if (!p) { set_error("null params"); return MHIT_ERR_ARG; }
+    if (!fft_size_supported(p->nx)) {
+        set_error("nx=" + std::to_string(p->nx) + " unsupported (2^a 3^b, 8..512)");
+        return MHIT_ERR_ARG;
+    }
+    if (!(p->dt > 0) || !(p->rho > 0) || !(p->epsr > 0) || p->mu < 0) {
+        set_error("dt, rho, epsr must be > 0 and mu >= 0");
+        return MHIT_ERR_ARG;
+    }
+    if (p->poisson_precision != MHIT_POISSON_FP32 && p->poisson_precision != MHIT_POISSON_FP64) {
+        set_error("poisson_precision must be MHIT_POISSON_FP32 or MHIT_POISSON_FP64");
+        return MHIT_ERR_ARG;
+    }
+    return MHIT_OK;
+}
+
+static int sync_scalars(mhit_ctx* c) {
+    MHIT_HIP_TRY(hipMemcpyAsync(&c->S_host, c->S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+static int push_scalars(mhit_ctx* c) {
+    MHIT_HIP_TRY(hipMemcpyAsync(c->S, &c->S_host, sizeof(DevScalars), hipMemcpyHostToDevice, c->stream));
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+static double* state_field(mhit_ctx* c, int fieldn) {
+    switch (fieldn) {
+        case MHIT_FIELD_U: return c->u;
+        case MHIT_FIELD_V: return c->v;
+        case MHIT_FIELD_W: return c->w;
+        case MHIT_FIELD_PHI: return c->phi;
+        default: return nullptr;
+    }
+}
+
+// one timestep, main.f90:456-1007
+static int enqueue_step(mhit_ctx* c) {
+    int rc;
+    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
+    if (c->P.phiflag && (rc = launch_acdi(c))) return rc;                    // 521-637
+    if ((rc = launch_ns(c))) return rc;                                      // 658-796
+    if ((rc = poisson_project(c))) return rc;                                // 833-923
+    if ((rc = launch_correct(c))) return rc;                                 // 942-962
+    if ((rc = launch_finalize(c))) return rc;                                // 964-1007
+    return MHIT_OK;
+}
+
+static int ensure_tracers(mhit_ctx* c, int64_t n) {
+    if (n <= c->np_cap) { c->np = n; return MHIT_OK; }
+    double** cols[] = {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1};
+    for (double** col : cols) { dfree(*col); *col = nullptr; }
+    int rc;
+    for (double** col : cols)
+        if ((rc = dmalloc(col, (size_t)n))) return rc;
+    c->np_cap = n;
+    c->np = n;
+    return MHIT_OK;
+}
+
+}  // namespace mhit
+
+using namespace mhit;
+
+extern "C" {
+
+const char* mhit_last_error(void) { return g_last_error.c_str(); }
+
+const char* mhit_version(void) { return "mhit36-mi355x 0.1 (gfx950)"; }
+
+int mhit_params_default(mhit_params* p) {
+    if (!p) return MHIT_ERR_ARG;
+    std::memset(p, 0, sizeof(*p));
+    // multi/input.inp:1-18
+    p->restart = 0; p->tstart = 0; p->tfin = 10000; p->dump = 100000;
+    p->inflow = 0; p->inphi = 0;
+    p->dt = 0.0002; p->mu = 0.006; p->rho = 1.0;
+    p->f1 = 1.0; p->f2 = 1.0; p->f3 = 1.0; p->k0 = 2.0;
+    p->radius = 0.5; p->sigma = 1.0; p->epsr = 1.0;
+    p->npart = 10000; p->inpart = 1;
+    p->nx = 256;                     // module.f90:2
+    p->phiflag = 0; p->partflag = 0; // main.f90:55-57
+    p->poisson_precision = MHIT_POISSON_FP32;
+    p->k2_mode = MHIT_K2_CONTINUOUS;
+    p->interp_mode = MHIT_INTERP_REFERENCE;
+    p->cfl_abort = 1;
+    return MHIT_OK;
+}
+
+int mhit_create(const mhit_params* P, mhit_ctx** out) {
+    if (!out) { set_error("null output"); return MHIT_ERR_ARG; }
+    *out = nullptr;
+    int rc = check_params(P);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return MHIT_ERR_NOGPU;
+    }
+    mhit_ctx* c = new mhit_ctx();
+    c->P = *P;
+    const int n = c->n = P->nx;
+    c->N3 = (size_t)n * n * n;
+    c->fp32 = P->poisson_precision == MHIT_POISSON_FP32;
+    // readinput.f90:39-49
+    const double twopi = 8.0 * std::atan(1.0);
+    Grid& g = c->g;
+    g.n = n;
+    g.lx = twopi;
+    g.dx = g.lx / (double)n;
+    g.dxi = 1.0 / g.dx;
+    g.ddxi = 1.0 / g.dx / g.dx;
+    g.rho = P->rho;
+    g.rhoi = 1.0 / P->rho;
+    g.mu = P->mu;
+    g.dt = P->dt;
+    g.eps = P->epsr * g.dx;
+    g.epsi = 1.0 / g.eps;
+    g.enm = (double)1.e-16f;   // `enum=1.e-16`: a default-real literal
+    g.sigma = P->sigma;
+    g.f1 = P->f1; g.f2 = P->f2; g.f3 = P->f3;
+
+    auto fail = [&](int code) { mhit_destroy(c); return code; };
+    if (hipGetDevice(&c->device) != hipSuccess) { set_error("hipGetDevice failed"); return fail(MHIT_ERR_HIP); }
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        return fail(MHIT_ERR_HIP);
+    }
+    const size_t N3 = c->N3;
+    double** fields[] = {&c->u, &c->v, &c->w, &c->us, &c->vs, &c->ws, &c->rhsu_o, &c->rhsv_o, &c->rhsw_o};
+    for (double** f : fields)
+        if ((rc = dmalloc(f, N3))) return fail(rc);
+    if (P->phiflag) {
+        double** pf[] = {&c->phi, &c->rhsphi, &c->rhsphi_o, &c->psi, &c->tanh_psi, &c->normx,
+                         &c->normy, &c->normz, &c->fxst, &c->fyst, &c->fzst};
+        for (double** f : pf)
+            if ((rc = dmalloc(f, N3))) return fail(rc);
+    }
+    const size_t H = n / 2;
+    const size_t csz = c->fp32 ? sizeof(float2) : sizeof(double2);
+    const size_t rsz = c->fp32 ? sizeof(float) : sizeof(double);
+    if ((rc = dmalloc((char**)&c->pres, N3 * rsz))) return fail(rc);
+    if ((rc = dmalloc((char**)&c->spec, (size_t)n * n * H * csz))) return fail(rc);
+    if ((rc = dmalloc((char**)&c->nyq, (size_t)n * n * csz))) return fail(rc);
+    if ((rc = dmalloc((char**)&c->tw, (size_t)n * csz))) return fail(rc);
+    if ((rc = dmalloc(&c->kk, (size_t)n))) return fail(rc);
+    if ((rc = dmalloc(&c->mysin, 3 * (size_t)(n + 1)))) return fail(rc);
+    c->mycos = c->mysin + (n + 1);
+    c->xext = c->mysin + 2 * (n + 1);
+    c->npart_blocks = correct_block_count(n);
+    if ((rc = dmalloc(&c->partials, (size_t)c->npart_blocks * 6))) return fail(rc);
+    if ((rc = dmalloc(&c->dpartials, (size_t)c->npart_blocks * 8))) return fail(rc);
+    if ((rc = dmalloc(&c->S, 1))) return fail(rc);
+
+    // host tables: grid by cumulative sum and x_ext(n+1) = lx (main.f90:210-217),
+    // wavenumbers (234-240), forcing tables (244-250) on the extended grid
+    std::vector<double> tab(3 * (size_t)(n + 1));
+    double* xe = tab.data() + 2 * (n + 1);
+    xe[0] = 0.0;
+    for (int i = 1; i < n; i++) xe[i] = xe[i - 1] + g.dx;
+    xe[n] = g.lx;
+    for (int t = 0; t <= n; t++) {
+        tab[t] = std::sin(P->k0 * xe[t] + g.dx / 2);
+        tab[(n + 1) + t] = std::cos(P->k0 * xe[t] + g.dx / 2);
+    }
+    std::vector<double> kx(n), kk(n);
+    for (int i = 1; i <= n / 2; i++) kx[i - 1] = (double)(i - 1) * (twopi / g.lx);
+    for (int i = n / 2 + 1; i <= n; i++) kx[i - 1] = (double)(i - 1 - n) * (twopi / g.lx);
+    for (int i = 0; i < n; i++) {
+        if (P->k2_mode == MHIT_K2_MODIFIED) {
+            const double s = 2.0 * std::sin(kx[i] * g.dx * 0.5) * g.dxi;
+            kk[i] = s * s;
+        } else {
+            kk[i] = kx[i] * kx[i];
+        }
+    }
+    std::vector<double2> twd(n);
+    std::vector<float2> twf(n);
+    for (int j = 0; j < n; j++) {
+        const double a = twopi * (double)j / (double)n;
+        twd[j].x = std::cos(a); twd[j].y = -std::sin(a);
+        twf[j].x = (float)twd[j].x; twf[j].y = (float)twd[j].y;
+    }
+    if (hipMemcpy(c->mysin, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->kk, kk.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->tw, c->fp32 ? (const void*)twf.data() : (const void*)twd.data(), n * csz,
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("table upload failed");
+        return fail(MHIT_ERR_HIP);
+    }
+    // zero state; AB2 start of main.f90:440-444
+    for (double** f : fields)
+        if (hipMemset(*f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
+    if (P->phiflag) {
+        double** pf[] = {&c->phi, &c->rhsphi, &c->rhsphi_o, &c->psi, &c->tanh_psi, &c->normx,
+                         &c->normy, &c->normz, &c->fxst, &c->fyst, &c->fzst};
+        for (double** f : pf)
+            if (hipMemset(*f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
+    }
+    if (hipMemset(c->pres, 0, N3 * rsz) != hipSuccess) return fail(MHIT_ERR_HIP);
+    c->S_host = DevScalars{};
+    c->S_host.alpha = 1.0;
+    c->S_host.beta = 0.0;
+    c->S_host.gumax = 1.0;
+    if ((rc = push_scalars(c))) return fail(rc);
+    *out = c;
+    return MHIT_OK;
+}
+
+int mhit_destroy(mhit_ctx* c) {
+    if (!c) return MHIT_OK;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void* ptrs[] = {c->u, c->v, c->w, c->us, c->vs, c->ws, c->rhsu_o, c->rhsv_o, c->rhsw_o,
+                    c->phi, c->rhsphi, c->rhsphi_o, c->psi, c->tanh_psi, c->normx, c->normy,
+                    c->normz, c->fxst, c->fyst, c->fzst, c->pres, c->spec, c->nyq, c->tw, c->kk,
+                    c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
+                    c->pu1, c->pv1, c->pw1};
+    for (void* p : ptrs) dfree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MHIT_OK;
+}
+
+int mhit_upload_field(mhit_ctx* c, int fieldn, const double* host) {
+    if (!c || !host) { set_error("null argument"); return MHIT_ERR_ARG; }
+    int rc;
+    const size_t N3 = c->N3;
+    if (fieldn == MHIT_FIELD_P) {
+        if (c->fp32) {
+            std::vector<float> tmp(N3);
+            for (size_t q = 0; q < N3; q++) tmp[q] = (float)host[q];
+            MHIT_HIP_TRY(hipMemcpy(c->pres, tmp.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            MHIT_HIP_TRY(hipMemcpy(c->pres, host, N3 * sizeof(double), hipMemcpyHostToDevice));
+        }
+        return MHIT_OK;
+    }
+    double* f = state_field(c, fieldn);
+    if (!f) { set_error("field not allocated (phiflag=0?) or bad fieldn"); return MHIT_ERR_ARG; }
+    if ((rc = sync_scalars(c))) return rc;
+    MHIT_HIP_TRY(hipMemcpy(f, host, N3 * sizeof(double), hipMemcpyHostToDevice));
+    if (fieldn <= MHIT_FIELD_W) {
+        c->S_host.mean[fieldn - 1] = 0.0;
+        if ((rc = push_scalars(c))) return rc;
+    }
+    return MHIT_OK;
+}
+
+int mhit_download_field(mhit_ctx* c, int fieldn, double* host) {
+    if (!c || !host) { set_error("null argument"); return MHIT_ERR_ARG; }
+    int rc;
+    const size_t N3 = c->N3;
+    if ((rc = sync_scalars(c))) return rc;
+    if (fieldn == MHIT_FIELD_P) {
+        if (c->fp32) {
+            std::vector<float> tmp(N3);
+            MHIT_HIP_TRY(hipMemcpy(tmp.data(), c->pres, N3 * sizeof(float), hipMemcpyDeviceToHost));
+            for (size_t q = 0; q < N3; q++) host[q] = (double)tmp[q];
+        } else {
+            MHIT_HIP_TRY(hipMemcpy(host, c->pres, N3 * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        return MHIT_OK;
+    }
+    double* f = state_field(c, fieldn);
+    if (!f) { set_error("field not allocated (phiflag=0?) or bad fieldn"); return MHIT_ERR_ARG; }
+    MHIT_HIP_TRY(hipMemcpy(host, f, N3 * sizeof(double), hipMemcpyDeviceToHost));
+    if (fieldn <= MHIT_FIELD_W) {
+        // main.f90:977-981 applied on the way out (the device keeps u + mean)
+        const double m = c->S_host.mean[fieldn - 1];
+        if (m != 0.0)
+            for (size_t q = 0; q < N3; q++) host[q] = host[q] - m;
+    }
+    return MHIT_OK;
+}
+
+int mhit_upload_tracers(mhit_ctx* c, int64_t n, const double* x, const double* y, const double* z) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z))) { set_error("bad tracer arguments"); return MHIT_ERR_ARG; }
+    int rc;
+    if ((rc = ensure_tracers(c, n))) return rc;
+    if (n == 0) return MHIT_OK;
+    std::vector<double> tag(n);
+    for (int64_t i = 0; i < n; i++) tag[i] = (double)(i + 1);   // particles.f90:46-49
+    const size_t b = (size_t)n * sizeof(double);
+    MHIT_HIP_TRY(hipMemcpy(c->ptag, tag.data(), b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(hipMemcpy(c->px, x, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(hipMemcpy(c->py, y, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(hipMemcpy(c->pz, z, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(hipMemset(c->pu1, 0, b));
+    MHIT_HIP_TRY(hipMemset(c->pv1, 0, b));
+    MHIT_HIP_TRY(hipMemset(c->pw1, 0, b));
+    return MHIT_OK;
+}
+
+int mhit_download_tracers(mhit_ctx* c, double* tag, double* x, double* y, double* z, double* up,
+                          double* vp, double* wp) {
+    if (!c) { set_error("null context"); return MHIT_ERR_ARG; }
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    const size_t b = (size_t)c->np * sizeof(double);
+    if (b == 0) return MHIT_OK;
+    struct { double* h; double* d; } cols[] = {{tag, c->ptag}, {x, c->px}, {y, c->py}, {z, c->pz},
+                                               {up, c->pu1}, {vp, c->pv1}, {wp, c->pw1}};
+    for (auto& col : cols)
+        if (col.h) MHIT_HIP_TRY(hipMemcpy(col.h, col.d, b, hipMemcpyDeviceToHost));
+    return MHIT_OK;
+}
+
+int64_t mhit_tracer_count(const mhit_ctx* c) { return c ? c->np : -1; }
+
+int mhit_step(mhit_ctx* c, int nsteps) {
+    if (!c || nsteps < 0) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    int rc;
+    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+    for (int s = 0; s < nsteps; s++)
+        if ((rc = enqueue_step(c))) return rc;
+    if ((rc = sync_scalars(c))) return rc;
+    if (c->S_host.halted) {
+        set_error("cou = " + std::to_string(c->S_host.cou) + " > 7 at step " +
+                  std::to_string(c->S_host.step) + " (main.f90:1006)");
+        return MHIT_ERR_CFL;
+    }
+    return MHIT_OK;
+}
+
+int mhit_set_ab2_state(mhit_ctx* c, double alpha, double beta, double gumax) {
+    if (!c) return MHIT_ERR_ARG;
+    int rc;
+    if ((rc = sync_scalars(c))) return rc;
+    c->S_host.alpha = alpha;
+    c->S_host.beta = beta;
+    c->S_host.gumax = gumax;
+    c->S_host.halted = 0;
+    return push_scalars(c);
+}
+
+int mhit_sync(mhit_ctx* c) {
+    if (!c) return MHIT_ERR_ARG;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+int mhit_diag_compute(mhit_ctx* c, mhit_diag* d) {
+    if (!c || !d) { set_error("null argument"); return MHIT_ERR_ARG; }
+    int rc;
+    if ((rc = sync_scalars(c))) return rc;
+    double r[8];
+    if ((rc = launch_diag(c, r))) return rc;
+    const DevScalars& S = c->S_host;
+    std::memset(d, 0, sizeof(*d));
+    d->step = S.step;
+    d->alpha = S.alpha; d->beta = S.beta; d->gumax = S.gumax; d->cou = S.cou;
+    d->umean = S.mean[0]; d->vmean = S.mean[1]; d->wmean = S.mean[2];
+    const double N3 = (double)c->N3;
+    d->ke = 0.5 * r[0] / N3;
+    d->dissipation = 2.0 * (c->P.mu / c->P.rho) * r[1] / N3;
+    d->maxdiv = r[2];
+    d->phi_mass = c->P.phiflag ? r[3] * c->g.dx * c->g.dx * c->g.dx : 0.0;
+    d->phi_min = c->P.phiflag ? r[5] : 0.0;
+    d->phi_max = c->P.phiflag ? r[6] : 0.0;
+    d->nan_count = (int64_t)r[7];
+    return MHIT_OK;
+}
+
+// ------------------------------------------------------------ initial conditions
+int mhit_init_taylor_green(mhit_ctx* c) {
+    if (!c) return MHIT_ERR_ARG;
+    int rc;
+    if ((rc = launch_init_tgv(c))) return rc;
+    if ((rc = sync_scalars(c))) return rc;
+    c->S_host.mean[0] = c->S_host.mean[1] = c->S_host.mean[2] = 0.0;
+    return push_scalars(c);
+}
+
+int mhit_init_drop(mhit_ctx* c) {
+    if (!c) return MHIT_ERR_ARG;
+    if (!c->P.phiflag) { set_error("phiflag = 0: no phase field"); return MHIT_ERR_STATE; }
+    int rc;
+    if ((rc = launch_init_drop(c))) return rc;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+int mhit_init_random_velocity(mhit_ctx* c, uint64_t seed, double kmax, double urms) {
+    if (!c || !(kmax >= 1.0)) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    const int n = c->n;
+    const double kme = std::fmin(kmax, (double)(n / 2 - 1));
+    const int K = (int)std::floor(kme);
+    const double twopi = 8.0 * std::atan(1.0);
+    std::vector<double> modes;
+    double ms = 0.0;
+    const uint64_t base = splitmix64(seed ^ 0x5EEDull);
+    for (int mz = -K; mz <= K; mz++)
+        for (int my = -K; my <= K; my++)
+            for (int mx = -K; mx <= K; mx++) {
+                const int m2 = mx * mx + my * my + mz * mz;
+                if (m2 == 0 || (double)m2 > kme * kme) continue;
+                if (!(mz > 0 || (mz == 0 && my > 0) || (mz == 0 && my == 0 && mx > 0))) continue;
+                // six normal deviates from a counter keyed by the mode
+                const uint64_t key = splitmix64(base + (uint64_t)((mx + 64) + 128 * ((my + 64) + 128 * (mz + 64))));
+                double gsn[6];
+                for (int t = 0; t < 3; t++) {
+                    double a = u01(splitmix64(key + 2 * t)), b = u01(splitmix64(key + 2 * t + 1));
+                    if (a < 1e-300) a = 1e-300;
+                    const double r = std::sqrt(-2.0 * std::log(a));
+                    gsn[2 * t] = r * std::cos(twopi * b);
+                    gsn[2 * t + 1] = r * std::sin(twopi * b);
+                }
+                const double amp = 1.0 / (double)m2;   // |A| ~ |m|^-2
+                double Ar[3], Ai[3], cr[3], ci[3];
+                const int md[3] = {mx, my, mz};
+                double cc = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    Ar[d] = amp * gsn[2 * d];
+                    Ai[d] = amp * gsn[2 * d + 1];
+                    const double th = twopi * (double)md[d] / (double)n;
+                    cr[d] = std::cos(th) - 1.0;
+                    ci[d] = std::sin(th);
+                    cc += cr[d] * cr[d] + ci[d] * ci[d];
+                }
+                // remove the component along conj(c): sum_d A_d c_d = 0 makes the
+                // forward-difference divergence of the mode vanish
+                double dr = 0.0, di = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    dr += Ar[d] * cr[d] - Ai[d] * ci[d];
+                    di += Ar[d] * ci[d] + Ai[d] * cr[d];
+                }
+                for (int d = 0; d < 3; d++) {
+                    // (dr + i di) * conj(c_d) / cc
+                    Ar[d] -= (dr * cr[d] + di * ci[d]) / cc;
+                    Ai[d] -= (di * cr[d] - dr * ci[d]) / cc;
+                }
+                modes.push_back(mx); modes.push_back(my); modes.push_back(mz);
+                for (int d = 0; d < 3; d++) {
+                    modes.push_back(2.0 * Ar[d]);
+                    modes.push_back(2.0 * Ai[d]);
+                    ms += 2.0 * (Ar[d] * Ar[d] + Ai[d] * Ai[d]);
+                }
+            }
+    const int nmodes = (int)(modes.size() / 9);
+    if (nmodes == 0 || ms <= 0.0) { set_error("no modes below kmax"); return MHIT_ERR_ARG; }
+    const double scale = urms / std::sqrt(ms / 3.0);
+    for (int m = 0; m < nmodes; m++)
+        for (int t = 3; t < 9; t++) modes[(size_t)m * 9 + t] *= scale;
+    std::vector<double2> E(n);
+    for (int t = 0; t < n; t++) {
+        E[t].x = std::cos(twopi * t / n);
+        E[t].y = std::sin(twopi * t / n);
+    }
+    double* dmodes = nullptr;
+    double2* dE = nullptr;
+    int rc;
+    if ((rc = dmalloc(&dmodes, modes.size()))) return rc;
+    if ((rc = dmalloc(&dE, (size_t)n))) { dfree(dmodes); return rc; }
+    hipError_t e1 = hipMemcpy(dmodes, modes.data(), modes.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e2 = hipMemcpy(dE, E.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    rc = (e1 == hipSuccess && e2 == hipSuccess) ? launch_init_modes(c, dmodes, nmodes, dE) : MHIT_ERR_HIP;
+    if (rc == MHIT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
+    dfree(dmodes);
+    dfree(dE);
+    if (rc) { set_error("random velocity init failed"); return rc; }
+    if ((rc = sync_scalars(c))) return rc;
+    c->S_host.mean[0] = c->S_host.mean[1] = c->S_host.mean[2] = 0.0;
+    return push_scalars(c);
+}
+
+int mhit_drop_centers(int32_t ndrops, double radius, double min_gap, uint64_t seed, double* xyz) {
+    if (ndrops < 0 || !(radius > 0) || (ndrops > 0 && !xyz)) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    const double L = 8.0 * std::atan(1.0);
+    const double dmin = 2.0 * radius + min_gap;
+    uint64_t ctr = splitmix64(seed ^ 0xD50Bull);
+    for (int d = 0; d < ndrops; d++) {
+        bool ok = false;
+        for (int attempt = 0; attempt < 100000 && !ok; attempt++) {
+            double c3[3];
+            for (int t = 0; t < 3; t++) c3[t] = u01(splitmix64(ctr++)) * L;
+            ok = true;
+            for (int e = 0; e < d && ok; e++) {
+                double r2 = 0.0;
+                for (int t = 0; t < 3; t++) {
+                    double a = std::fabs(c3[t] - xyz[3 * e + t]);
+                    a = std::fmin(a, L - a);
+                    r2 += a * a;
+                }
+                if (r2 < dmin * dmin) ok = false;
+            }
+            if (ok)
+                for (int t = 0; t < 3; t++) xyz[3 * d + t] = c3[t];
+        }
+        if (!ok) { set_error("could not place " + std::to_string(ndrops) + " drops without overlap"); return MHIT_ERR_ARG; }
+    }
+    return MHIT_OK;
+}
+
+int mhit_init_drops(mhit_ctx* c, int32_t ndrops, double radius, uint64_t seed) {
+    if (!c) return MHIT_ERR_ARG;
+    if (!c->P.phiflag) { set_error("phiflag = 0: no phase field"); return MHIT_ERR_STATE; }
+    std::vector<double> xyz(3 * (size_t)(ndrops > 0 ? ndrops : 1));
+    int rc;
+    if ((rc = mhit_drop_centers(ndrops, radius, 4.0 * c->g.eps, seed, xyz.data()))) return rc;
+    double* dc = nullptr;
+    if ((rc = dmalloc(&dc, xyz.size()))) return rc;
+    rc = hipMemcpy(dc, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+             ? launch_init_drops(c, dc, ndrops, radius) : MHIT_ERR_HIP;
+    if (rc == MHIT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
+    dfree(dc);
+    return rc;
+}
+
+int mhit_init_tracers_uniform(mhit_ctx* c, int64_t n, uint64_t seed) {
+    if (!c || n < 0) return MHIT_ERR_ARG;
+    int rc;
+    if ((rc = ensure_tracers(c, n))) return rc;
+    if (n == 0) return MHIT_OK;
+    if ((rc = launch_init_tracers(c, n, seed))) return rc;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+// ------------------------------------------------------------ building blocks
+int mhit_poisson_solve(mhit_ctx* c, const double* rhs, double* p) {
+    if (!c || !rhs || !p) { set_error("null argument"); return MHIT_ERR_ARG; }
+    double* d = nullptr;
+    int rc;
+    if ((rc = dmalloc(&d, c->N3))) return rc;
+    if (hipMemcpy(d, rhs, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
+    if (!rc) rc = poisson_solve_field(c, d);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
+    dfree(d);
+    if (rc) return rc;
+    return mhit_download_field(c, MHIT_FIELD_P, p);
+}
+
+}  // extern "C"
+
+namespace mhit {
+
+template <typename V>
+static int download_spectrum(mhit_ctx* c, double* out) {
+    const int n = c->n, H = n / 2;
+    std::vector<V> spec((size_t)n * n * H), nyq((size_t)n * n);
+    MHIT_HIP_TRY(hipMemcpy(spec.data(), c->spec, spec.size() * sizeof(V), hipMemcpyDeviceToHost));
+    MHIT_HIP_TRY(hipMemcpy(nyq.data(), c->nyq, nyq.size() * sizeof(V), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < (size_t)n * n; r++) {
+        for (int k = 0; k < H; k++) {
+            out[2 * (r * (H + 1) + k)] = (double)spec[r * H + k].x;
+            out[2 * (r * (H + 1) + k) + 1] = (double)spec[r * H + k].y;
+        }
+        out[2 * (r * (H + 1) + H)] = (double)nyq[r].x;
+        out[2 * (r * (H + 1) + H) + 1] = (double)nyq[r].y;
+    }
+    return MHIT_OK;
+}
+
+template <typename V>
+static int upload_spectrum(mhit_ctx* c, const double* in) {
+    const int n = c->n, H = n / 2;
+    std::vector<V> spec((size_t)n * n * H), nyq((size_t)n * n);
+    for (size_t r = 0; r < (size_t)n * n; r++) {
+        for (int k = 0; k < H; k++) {
+            spec[r * H + k].x = in[2 * (r * (H + 1) + k)];
+            spec[r * H + k].y = in[2 * (r * (H + 1) + k) + 1];
+        }
+        nyq[r].x = in[2 * (r * (H + 1) + H)];
+        nyq[r].y = in[2 * (r * (H + 1) + H) + 1];
+    }
+    MHIT_HIP_TRY(hipMemcpy(c->spec, spec.data(), spec.size() * sizeof(V), hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(hipMemcpy(c->nyq, nyq.data(), nyq.size() * sizeof(V), hipMemcpyHostToDevice));
+    return MHIT_OK;
+}
+
+}  // namespace mhit
+
+extern "C" {
+
+int mhit_fft_r2c(mhit_ctx* c, const double* in, double* out) {
+    if (!c || !in || !out) { set_error("null argument"); return MHIT_ERR_ARG; }
+    double* d = nullptr;
+    int rc;
+    if ((rc = dmalloc(&d, c->N3))) return rc;
+    if (hipMemcpy(d, in, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
+    if (!rc) rc = fft_r2c_dev(c, d);
+    if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
+    dfree(d);
+    if (rc) return rc;
+    return c->fp32 ? download_spectrum<float2>(c, out) : download_spectrum<double2>(c, out);
+}
+
+int mhit_fft_c2r(mhit_ctx* c, const double* in, double* out) {
+    if (!c || !in || !out) { set_error("null argument"); return MHIT_ERR_ARG; }
+    int rc = c->fp32 ? upload_spectrum<float2>(c, in) : upload_spectrum<double2>(c, in);
+    if (rc) return rc;
+    if ((rc = fft_c2r_dev(c))) return rc;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return mhit_download_field(c, MHIT_FIELD_P, out);
+}
+
+}  // extern "C"

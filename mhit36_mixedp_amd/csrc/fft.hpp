@@ -1,0 +1,172 @@
+// fft.hpp -- batched Stockham FFT engine in LDS for gfx950.
+//
+// A workgroup holds C independent length-N complex columns in LDS, laid
+// out [a][LD] (element a of column c at a*LD + c), and transforms them in
+// place with TPC threads per column (thread id = t*C + c, so the C threads
+// that share t touch C adjacent words: bank-conflict free).  Stages are
+// radix-3/8/4/2 Stockham autosort steps (natural order in and out):
+//   stage of radix R after NS points are done, butterfly b in [0, N/R):
+//     k = b mod NS, inputs x[b + r N/R] * W_{NS R}^{r k},
+//     DFT_R, outputs y[(b div NS) NS R + k + r NS].
+// Twiddles come from a per-N table W_N^j = exp(-2 pi i j/N) computed on the
+// host in double; the inverse uses the conjugate.  Transforms are
+// unnormalised, forward e^{-i}, inverse e^{+i}, as cuFFT's used by the
+// reference (multi/main.f90:853-912).
+#pragma once
+
+#include "common.hpp"
+
+namespace mhit {
+
+template <typename T> struct CplxOf;
+template <> struct CplxOf<float> { using type = float2; };
+template <> struct CplxOf<double> { using type = double2; };
+template <typename T> using cplx_t = typename CplxOf<T>::type;
+
+template <typename V> __device__ __forceinline__ V c_add(V a, V b) { V r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename V> __device__ __forceinline__ V c_sub(V a, V b) { V r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename V> __device__ __forceinline__ V c_mul(V a, V b) {
+    V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+// multiply by -i (forward) or +i (inverse)
+template <bool INV, typename V> __device__ __forceinline__ V c_rot(V a) {
+    V r;
+    if (INV) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+    return r;
+}
+
+// Radix plan: factors of 3 first, then 8s (16 -> 4*4), then a 4 or a 2.
+template <int REM> struct NextRadix {
+    static constexpr int value = (REM % 3 == 0) ? 3
+                               : (REM % 8 == 0 && REM != 16) ? 8
+                               : (REM % 4 == 0) ? 4 : 2;
+};
+
+template <bool INV, typename V> __device__ __forceinline__ void dft2(V* a) {
+    V t = a[0];
+    a[0] = c_add(t, a[1]);
+    a[1] = c_sub(t, a[1]);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft3(V* a) {
+    using T = decltype(a[0].x);
+    const T s = (T)0.86602540378443864676;
+    V t1 = c_add(a[1], a[2]);
+    V t2 = c_sub(a[1], a[2]);
+    V m1; m1.x = a[0].x - (T)0.5 * t1.x; m1.y = a[0].y - (T)0.5 * t1.y;
+    V st; st.x = s * t2.x; st.y = s * t2.y;
+    V r = c_rot<INV>(st);
+    a[0] = c_add(a[0], t1);
+    a[1] = c_add(m1, r);
+    a[2] = c_sub(m1, r);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft4v(V& a0, V& a1, V& a2, V& a3) {
+    V s02 = c_add(a0, a2), d02 = c_sub(a0, a2);
+    V s13 = c_add(a1, a3), d13 = c_rot<INV>(c_sub(a1, a3));
+    a0 = c_add(s02, s13);
+    a2 = c_sub(s02, s13);
+    a1 = c_add(d02, d13);
+    a3 = c_sub(d02, d13);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft4(V* a) {
+    dft4v<INV>(a[0], a[1], a[2], a[3]);
+}
+
+template <bool INV, typename V> __device__ __forceinline__ void dft8(V* a) {
+    using T = decltype(a[0].x);
+    const T c = (T)0.70710678118654752440;
+    V e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+    V o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+    dft4v<INV>(e0, e1, e2, e3);
+    dft4v<INV>(o0, o1, o2, o3);
+    V t1, t3;
+    if (!INV) {  // W8^1 = c(1-i), W8^3 = -c(1+i)
+        t1.x = c * (o1.x + o1.y); t1.y = c * (o1.y - o1.x);
+        t3.x = c * (o3.y - o3.x); t3.y = -c * (o3.x + o3.y);
+    } else {     // conjugates
+        t1.x = c * (o1.x - o1.y); t1.y = c * (o1.x + o1.y);
+        t3.x = -c * (o3.x + o3.y); t3.y = c * (o3.x - o3.y);
+    }
+    V t2 = c_rot<INV>(o2);
+    a[0] = c_add(e0, o0); a[4] = c_sub(e0, o0);
+    a[1] = c_add(e1, t1); a[5] = c_sub(e1, t1);
+    a[2] = c_add(e2, t2); a[6] = c_sub(e2, t2);
+    a[3] = c_add(e3, t3); a[7] = c_sub(e3, t3);
+}
+
+template <int R, bool INV, typename V> __device__ __forceinline__ void dft(V* a) {
+    if constexpr (R == 2) dft2<INV>(a);
+    else if constexpr (R == 3) dft3<INV>(a);
+    else if constexpr (R == 4) dft4<INV>(a);
+    else dft8<INV>(a);
+}
+
+// One Stockham stage on C columns in LDS.
+template <typename V, int N, int R, int NS, int C, int TPC, int LD, bool INV>
+__device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restrict__ tw) {
+    constexpr int NBF = N / R;
+    constexpr int NB = (NBF + TPC - 1) / TPC;
+    V v[NB][R];
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int b = t + q * TPC;
+        if (NBF % TPC == 0 || b < NBF) {
+#pragma unroll
+            for (int r = 0; r < R; r++) v[q][r] = lds[(b + r * NBF) * LD + c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int b = t + q * TPC;
+        if (NBF % TPC == 0 || b < NBF) {
+            const int k = b % NS;
+            if constexpr (NS > 1) {
+#pragma unroll
+                for (int r = 1; r < R; r++) {
+                    V w = tw[(r * k) * (N / (NS * R))];
+                    if (INV) w.y = -w.y;
+                    v[q][r] = c_mul(v[q][r], w);
+                }
+            }
+            dft<R, INV>(v[q]);
+            const int j = (b / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[(j + r * NS) * LD + c] = v[q][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename V, int N, int NS, int C, int TPC, int LD, bool INV>
+__device__ __forceinline__ void fft_stages(V* lds, int c, int t, const V* __restrict__ tw) {
+    if constexpr (NS < N) {
+        constexpr int R = NextRadix<N / NS>::value;
+        fft_stage<V, N, R, NS, C, TPC, LD, INV>(lds, c, t, tw);
+        fft_stages<V, N, NS * R, C, TPC, LD, INV>(lds, c, t, tw);
+    }
+}
+
+// ---------------------------------------------------------------- config
+constexpr int pow2_divisor_cap(int x, int cap) {
+    int c = 1;
+    while (c * 2 <= cap && x % (c * 2) == 0) c *= 2;
+    return c;
+}
+
+template <typename T, int N> struct FftCfg {
+    static constexpr int H = N / 2;
+    static constexpr bool F32 = sizeof(T) == 4;
+    // columns per workgroup for the strided (y, z) passes: 128-byte rows
+    static constexpr int CY = pow2_divisor_cap(H, F32 ? 16 : 8);
+    // columns per workgroup for the contiguous Nyquist-plane y pass
+    static constexpr int CN = pow2_divisor_cap(N, F32 ? 8 : 4);
+    // row pairs per workgroup for the x passes
+    static constexpr int PX = pow2_divisor_cap(H, F32 ? 8 : 4);
+    // threads per column
+    static constexpr int TPC = (N % 8 == 0) ? N / 8 : ((N % 4 == 0) ? N / 4 : N / 2);
+};
+
+}  // namespace mhit

@@ -1,0 +1,79 @@
+// internal.hpp -- context layout and launcher declarations shared by the
+// translation units of libmhit36.so.
+#pragma once
+
+#include "common.hpp"
+
+struct mhit_ctx {
+    mhit_params P;
+    int n = 0;
+    size_t N3 = 0;
+    mhit::Grid g{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool fp32 = true;
+
+    // velocity state (stored with the deferred mean, see DevScalars::mean)
+    double *u = nullptr, *v = nullptr, *w = nullptr;
+    // provisional velocity u* (main.f90:783-796)
+    double *us = nullptr, *vs = nullptr, *ws = nullptr;
+    double *rhsu_o = nullptr, *rhsv_o = nullptr, *rhsw_o = nullptr;
+    // phase field (module.f90:66-71)
+    double *phi = nullptr, *rhsphi = nullptr, *rhsphi_o = nullptr;
+    double *psi = nullptr, *tanh_psi = nullptr;
+    double *normx = nullptr, *normy = nullptr, *normz = nullptr;
+    double *fxst = nullptr, *fyst = nullptr, *fzst = nullptr;
+    // Poisson: pressure (FP32 or FP64), half spectrum [z][y][kx < n/2] and
+    // the kx = n/2 (Nyquist) plane [z][y], complex FP32 or FP64
+    void *pres = nullptr, *spec = nullptr, *nyq = nullptr;
+    void *tw = nullptr;           // W_n^j, j < n, complex in the Poisson precision
+    double *kk = nullptr;         // per-axis k^2 by wavenumber index (n)
+    // forcing tables and extended grid, one allocation of 3(n+1):
+    // mysin[0..n], mycos[0..n], xext[0..n] (main.f90:210-250)
+    double *mysin = nullptr, *mycos = nullptr, *xext = nullptr;
+    // reductions
+    double *partials = nullptr;   // correction partial sums/maxima
+    int npart_blocks = 0;
+    double *dpartials = nullptr;  // diagnostics partials
+    int ndiag_blocks = 0;
+    mhit::DevScalars *S = nullptr;
+    // tracers (particles.f90:11-22, SoA)
+    int64_t np = 0, np_cap = 0;
+    double *ptag = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
+    double *pu = nullptr, *pv = nullptr, *pw = nullptr;
+    double *pu1 = nullptr, *pv1 = nullptr, *pw1 = nullptr;
+    // host copy of scalars after the last sync
+    mhit::DevScalars S_host{};
+};
+
+namespace mhit {
+
+// Supported grid sizes (FFT lengths).
+bool fft_size_supported(int n);
+
+// poisson.hip
+enum class XSrc { Divergence, Real };
+int poisson_project(mhit_ctx* c);                       // u*,v*,w* -> pres
+int poisson_solve_field(mhit_ctx* c, const double* rhs_dev);  // rhs -> pres
+int fft_r2c_dev(mhit_ctx* c, const double* in_dev);     // -> spec, nyq
+int fft_c2r_dev(mhit_ctx* c);                           // spec, nyq -> pres
+
+// stencil.hip
+int launch_acdi(mhit_ctx* c);
+int launch_ns(mhit_ctx* c);
+int launch_correct(mhit_ctx* c);
+int launch_finalize(mhit_ctx* c);
+int launch_diag(mhit_ctx* c, double* host_out /* 8 values */);
+int correct_block_count(int n);
+
+// tracers.hip
+int launch_tracers(mhit_ctx* c);
+
+// init.hip
+int launch_init_tgv(mhit_ctx* c);
+int launch_init_drop(mhit_ctx* c);
+int launch_init_modes(mhit_ctx* c, const double* modes_dev, int nmodes, const double2* E);
+int launch_init_drops(mhit_ctx* c, const double* centers_dev, int ndrops, double radius);
+int launch_init_tracers(mhit_ctx* c, int64_t n, uint64_t seed);
+
+}  // namespace mhit

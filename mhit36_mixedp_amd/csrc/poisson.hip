@@ -1,0 +1,314 @@
+// poisson.hip -- FFT Poisson projection on one GPU (gfx950).
+//
+// Replaces the reference's cuFFT + cuDecomp pipeline (multi/main.f90:833-923):
+// divergence -> R2C x -> transpose -> C2C y -> transpose -> C2C z -> divide
+// by -k^2 -> inverse z -> transpose -> inverse y -> transpose -> C2R x.
+// On one GPU no transpose is needed: every pass reads the spectrum in
+// place, in the layout that keeps its own axis strided and 16 (FP32) or 8
+// (FP64) wavenumbers contiguous per row segment, and stages columns
+// through LDS.
+//
+// Layout in HBM (C order):
+//   spec[z][y][kx], kx < n/2   complex, the half spectrum without Nyquist
+//   nyq [z][y]                 complex, the kx = n/2 (Nyquist) plane
+//   pres[z][y][x]              real, the pressure
+// Passes per projection:
+//   X1 (x_fwd):   divergence of u* (main.f90:833-848) fused with the R2C
+//                 along x, two real rows per complex transform
+//   Y1, Y1n:      forward C2C along y on spec and on nyq
+//   Z:            forward C2C along z, divide (main.f90:883-896), inverse
+//   Y2, Y2n:      inverse C2C along y
+//   X2 (x_inv):   C2R along x, writes pres
+#include "fft.hpp"
+#include "internal.hpp"
+
+namespace mhit {
+
+// --------------------------------------------------------------- x forward
+// One workgroup: PX pairs of y rows at one z.  Row pair (ya, ya+1) is
+// transformed as one complex sequence a + i b and split with
+// X_a[k] = (Z[k] + conj Z[N-k])/2, X_b[k] = (Z[k] - conj Z[N-k])/(2i).
+template <typename T, int N, bool DIV>
+__global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
+void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
+                 const double* __restrict__ ws, double cdiv,
+                 cplx_t<T>* __restrict__ spec, cplx_t<T>* __restrict__ nyq,
+                 const cplx_t<T>* __restrict__ tw, const DevScalars* __restrict__ S) {
+    using V = cplx_t<T>;
+    using Cfg = FftCfg<T, N>;
+    constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
+    constexpr int NT = P * TPC;
+    __shared__ V lds[N * LD];
+    if (S && S->halted) return;
+    const int y0 = blockIdx.x * 2 * P;
+    const int z = blockIdx.y;
+    const int tid = threadIdx.x;
+    const int zp = (z + 1 == N) ? 0 : z + 1;
+
+    for (int e = tid; e < N * P; e += NT) {
+        const int a = e % N, c = e / N;
+        double d[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int y = y0 + 2 * c + h;
+            const size_t q = idx3(a, y, z, N);
+            if (DIV) {
+                // rhsp, main.f90:841-843
+                const int ap = (a + 1 == N) ? 0 : a + 1;
+                const int yp = (y + 1 == N) ? 0 : y + 1;
+                double r = cdiv * (us[idx3(ap, y, z, N)] - us[q]);
+                r = r + cdiv * (vs[idx3(a, yp, z, N)] - vs[q]);
+                r = r + cdiv * (ws[idx3(a, y, zp, N)] - ws[q]);
+                d[h] = r;
+            } else {
+                d[h] = us[q];
+            }
+        }
+        V val;
+        val.x = (T)d[0];   // rhspp = real(rhsp,4) in the FP32 form (main.f90:844)
+        val.y = (T)d[1];
+        lds[a * LD + c] = val;
+    }
+    __syncthreads();
+    fft_stages<V, N, 1, P, TPC, LD, false>(lds, tid % P, tid / P, tw);
+
+    for (int e = tid; e < H * 2 * P; e += NT) {
+        const int k = e % H, rr = e / H, cc = rr >> 1;
+        const V zk = lds[k * LD + cc];
+        const V zm = lds[((N - k) % N) * LD + cc];
+        V out;
+        if ((rr & 1) == 0) {
+            out.x = (zk.x + zm.x) * (T)0.5;
+            out.y = (zk.y - zm.y) * (T)0.5;
+        } else {
+            out.x = (zk.y + zm.y) * (T)0.5;
+            out.y = (zm.x - zk.x) * (T)0.5;
+        }
+        spec[((size_t)z * N + y0 + rr) * H + k] = out;
+    }
+    for (int rr = tid; rr < 2 * P; rr += NT) {
+        const V zh = lds[H * LD + (rr >> 1)];
+        V out;
+        out.x = (rr & 1) ? zh.y : zh.x;
+        out.y = (T)0;
+        nyq[(size_t)z * N + y0 + rr] = out;
+    }
+}
+
+// --------------------------------------------------------------- x inverse
+// C2R: Z[k] = X_a[k] + i X_b[k] over the Hermitian extension; the imaginary
+// parts of the kx = 0 and kx = n/2 coefficients are ignored (a real
+// transform has none).  Real part -> row ya, imaginary part -> row ya+1.
+template <typename T, int N>
+__global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
+void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict__ nyq,
+                 T* __restrict__ pres, const cplx_t<T>* __restrict__ tw,
+                 const DevScalars* __restrict__ S) {
+    using V = cplx_t<T>;
+    using Cfg = FftCfg<T, N>;
+    constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
+    constexpr int NT = P * TPC;
+    __shared__ V lds[N * LD];
+    if (S && S->halted) return;
+    const int y0 = blockIdx.x * 2 * P;
+    const int z = blockIdx.y;
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < H * P; e += NT) {
+        const int k = e % H, c = e / H;
+        const size_t ra = ((size_t)z * N + y0 + 2 * c) * H;
+        const V A = spec[ra + k];
+        const V B = spec[ra + H + k];
+        if (k == 0) {
+            V z0; z0.x = A.x; z0.y = B.x;
+            const V An = nyq[(size_t)z * N + y0 + 2 * c];
+            const V Bn = nyq[(size_t)z * N + y0 + 2 * c + 1];
+            V zh; zh.x = An.x; zh.y = Bn.x;
+            lds[c] = z0;
+            lds[H * LD + c] = zh;
+        } else {
+            V zk; zk.x = A.x - B.y; zk.y = A.y + B.x;
+            V zm; zm.x = A.x + B.y; zm.y = B.x - A.y;
+            lds[k * LD + c] = zk;
+            lds[(N - k) * LD + c] = zm;
+        }
+    }
+    __syncthreads();
+    fft_stages<V, N, 1, P, TPC, LD, true>(lds, tid % P, tid / P, tw);
+
+    for (int e = tid; e < N * 2 * P; e += NT) {
+        const int x = e % N, rr = e / N;
+        const V zz = lds[x * LD + (rr >> 1)];
+        pres[idx3(x, y0 + rr, z, N)] = (rr & 1) ? zz.y : zz.x;
+    }
+}
+
+// ------------------------------------------------------------ column pass
+// C columns of length N, element (c, a) at data[base + a*sa + c*sc] with
+// base = blockIdx.x*bx + blockIdx.y*by.  MODE 0 forward, 1 inverse,
+// 2 forward + spectral divide + inverse (the z pass).  AXIS_CONTIG picks
+// the global access order (a fastest when the axis is contiguous).
+// NYQ marks the Nyquist plane, where the columns are y wavenumbers and
+// kx = n/2; otherwise the columns are kx wavenumbers and blockIdx.y is ky.
+template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool NYQ>
+__global__ __launch_bounds__((C * FftCfg<T, N>::TPC))
+void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, size_t sc,
+                const cplx_t<T>* __restrict__ tw, const double* __restrict__ kk, double ntot,
+                const DevScalars* __restrict__ S) {
+    using V = cplx_t<T>;
+    constexpr int TPC = FftCfg<T, N>::TPC;
+    constexpr int LD = AXIS_CONTIG ? C + 1 : C;
+    constexpr int NT = C * TPC;
+    __shared__ V lds[N * LD];
+    if (S && S->halted) return;
+    const int tid = threadIdx.x;
+    V* __restrict__ g = data + blockIdx.x * bx + blockIdx.y * by;
+
+    for (int e = tid; e < N * C; e += NT) {
+        int a, c;
+        if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
+        lds[a * LD + c] = g[a * sa + c * sc];
+    }
+    __syncthreads();
+    const int c = tid % C, t = tid / C;
+    if (MODE == 0 || MODE == 2) fft_stages<V, N, 1, C, TPC, LD, false>(lds, c, t, tw);
+    else fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, tw);
+
+    if (MODE == 2) {
+        // main.f90:883-896: p^ = -r^/k2/(nx ny nz), k2 = kx^2+ky^2+kz^2
+        // in FP64, stored back in the spectrum precision; zero mode 0.
+        const int kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
+        const int kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+        const double kxy = kk[kxi] + kk[kyi];
+        for (int a = t; a < N; a += TPC) {
+            V val = lds[a * LD + c];
+            const double k2 = kxy + kk[a];
+            if (!NYQ && kxi == 0 && kyi == 0 && a == 0) {
+                val.x = (T)0; val.y = (T)0;
+            } else {
+                val.x = (T)(-(double)val.x / k2 / ntot);
+                val.y = (T)(-(double)val.y / k2 / ntot);
+            }
+            lds[a * LD + c] = val;
+        }
+        __syncthreads();
+        fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, tw);
+    }
+
+    for (int e = tid; e < N * C; e += NT) {
+        int a, cc;
+        if (AXIS_CONTIG) { a = e % N; cc = e / N; } else { cc = e % C; a = e / C; }
+        g[a * sa + cc * sc] = lds[a * LD + cc];
+    }
+}
+
+// ---------------------------------------------------------------- drivers
+template <typename T, int N>
+struct PoissonPasses {
+    using V = cplx_t<T>;
+    using Cfg = FftCfg<T, N>;
+    static constexpr int H = N / 2;
+
+    static void x_fwd(mhit_ctx* c, const double* a, const double* b, const double* w, bool div) {
+        dim3 grid(N / (2 * Cfg::PX), N), block(Cfg::PX * Cfg::TPC);
+        const double cdiv = c->g.rho * c->g.dxi / c->g.dt;
+        if (div)
+            k_fft_x_fwd<T, N, true><<<grid, block, 0, c->stream>>>(
+                a, b, w, cdiv, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
+        else
+            k_fft_x_fwd<T, N, false><<<grid, block, 0, c->stream>>>(
+                a, nullptr, nullptr, 0.0, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
+    }
+    template <int MODE>
+    static void y_pass(mhit_ctx* c) {
+        constexpr int CY = Cfg::CY, CN = Cfg::CN;
+        // spec: columns kx at fixed z; axis y (stride H)
+        k_fft_cols<T, N, CY, false, MODE, false>
+            <<<dim3(H / CY, N), dim3(CY * Cfg::TPC), 0, c->stream>>>(
+                (V*)c->spec, CY, (size_t)N * H, H, 1, (const V*)c->tw, c->kk, 0.0, c->S);
+        // nyq [z][y]: columns z, axis y contiguous
+        k_fft_cols<T, N, CN, true, MODE, true>
+            <<<dim3(N / CN, 1), dim3(CN * Cfg::TPC), 0, c->stream>>>(
+                (V*)c->nyq, (size_t)CN * N, 0, 1, N, (const V*)c->tw, c->kk, 0.0, c->S);
+    }
+    template <int MODE>
+    static void z_pass(mhit_ctx* c) {
+        constexpr int CY = Cfg::CY;
+        const double ntot = (double)((long long)N * N * N);
+        // spec: columns kx at fixed y (blockIdx.y); axis z (stride N*H)
+        k_fft_cols<T, N, CY, false, MODE, false>
+            <<<dim3(H / CY, N), dim3(CY * Cfg::TPC), 0, c->stream>>>(
+                (V*)c->spec, CY, H, (size_t)N * H, 1, (const V*)c->tw, c->kk, ntot, c->S);
+        // nyq [z][y]: columns y, axis z (stride N)
+        k_fft_cols<T, N, CY, false, MODE, true>
+            <<<dim3(N / CY, 1), dim3(CY * Cfg::TPC), 0, c->stream>>>(
+                (V*)c->nyq, CY, 0, N, 1, (const V*)c->tw, c->kk, ntot, c->S);
+    }
+    static void x_inv(mhit_ctx* c) {
+        dim3 grid(N / (2 * Cfg::PX), N), block(Cfg::PX * Cfg::TPC);
+        k_fft_x_inv<T, N><<<grid, block, 0, c->stream>>>(
+            (const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S);
+    }
+    static void project(mhit_ctx* c, const double* a, const double* b, const double* w, bool div) {
+        x_fwd(c, a, b, w, div);
+        y_pass<0>(c);
+        z_pass<2>(c);
+        y_pass<1>(c);
+        x_inv(c);
+    }
+    static void r2c(mhit_ctx* c, const double* in) {
+        x_fwd(c, in, nullptr, nullptr, false);
+        y_pass<0>(c);
+        z_pass<0>(c);
+    }
+    static void c2r(mhit_ctx* c) {
+        z_pass<1>(c);
+        y_pass<1>(c);
+        x_inv(c);
+    }
+};
+
+#define MHIT_FOR_EACH_N(X) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512)
+
+bool fft_size_supported(int n) {
+    switch (n) {
+#define CASE(NN) case NN:
+        MHIT_FOR_EACH_N(CASE)
+#undef CASE
+        return true;
+        default: return false;
+    }
+}
+
+enum class Op { Project, Solve, R2C, C2R };
+
+template <typename T>
+static int dispatch(mhit_ctx* c, Op op, const double* a, const double* b, const double* w) {
+    switch (c->n) {
+#define CASE(NN)                                                                   \
+    case NN:                                                                       \
+        if (op == Op::Project) PoissonPasses<T, NN>::project(c, a, b, w, true);   \
+        else if (op == Op::Solve) PoissonPasses<T, NN>::project(c, a, b, w, false);\
+        else if (op == Op::R2C) PoissonPasses<T, NN>::r2c(c, a);                  \
+        else PoissonPasses<T, NN>::c2r(c);                                         \
+        break;
+        MHIT_FOR_EACH_N(CASE)
+#undef CASE
+        default:
+            set_error("unsupported grid size for the FFT");
+            return MHIT_ERR_ARG;
+    }
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+static int run(mhit_ctx* c, Op op, const double* a, const double* b, const double* w) {
+    return c->fp32 ? dispatch<float>(c, op, a, b, w) : dispatch<double>(c, op, a, b, w);
+}
+
+int poisson_project(mhit_ctx* c) { return run(c, Op::Project, c->us, c->vs, c->ws); }
+int poisson_solve_field(mhit_ctx* c, const double* rhs) { return run(c, Op::Solve, rhs, nullptr, nullptr); }
+int fft_r2c_dev(mhit_ctx* c, const double* in) { return run(c, Op::R2C, in, nullptr, nullptr); }
+int fft_c2r_dev(mhit_ctx* c) { return run(c, Op::C2R, nullptr, nullptr, nullptr); }
+
+}  // namespace mhit

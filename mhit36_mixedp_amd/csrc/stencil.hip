@@ -1,0 +1,439 @@
+// stencil.hip -- finite-difference passes of the MHIT36 timestep (gfx950).
+//
+// Each kernel restates the reference loop it cites (aroccon/MHIT36_mixedp,
+// multi/main.f90) with the reference's operand order; the library is built
+// without FP contraction.  Halo exchanges are replaced by periodic index
+// wrap.  The velocity state is stored with a deferred mean: the reference
+// removes the mean from u,v,w after the correction (main.f90:977-981); here
+// the correction writes u + mean and every reader subtracts mean on load,
+// which yields the same FP64 values without a second pass over u,v,w.
+#include "internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace mhit {
+
+#define MHIT_CELL_INDEX(g)                                   \
+    const int n = (g).n;                                     \
+    const int i = blockIdx.x * 64 + threadIdx.x;             \
+    const int j = blockIdx.y * 4 + threadIdx.y;              \
+    const int k = blockIdx.z;                                \
+    if (i >= n || j >= n) return;                            \
+    const int ip = wrap_p(i, n), im = wrap_m(i, n);          \
+    const int jp = wrap_p(j, n), jm = wrap_m(j, n);          \
+    const int kp = wrap_p(k, n), km = wrap_m(k, n);          \
+    (void)ip; (void)im; (void)jp; (void)jm; (void)kp; (void)km; \
+    const size_t q = idx3(i, j, k, n)
+
+// ----------------------------------------------------------------- ACDI
+// P1, main.f90:522-534 (all points)
+__global__ __launch_bounds__(256)
+void k_acdi_psi(const double* __restrict__ phi, double* __restrict__ psi,
+                double* __restrict__ tpsi, Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double val = fmin(phi[q], 1.0);
+    const double ps = g.eps * log((val + g.enm) / (1.0 - val + g.enm));
+    psi[q] = ps;
+    tpsi[q] = tanh(0.5 * ps * g.epsi);
+}
+
+// P2, main.f90:536-564: convective + diffusive flux, raw normals
+__global__ __launch_bounds__(256)
+void k_acdi_rhs(const double* __restrict__ u, const double* __restrict__ v,
+                const double* __restrict__ w, const double* __restrict__ phi,
+                const double* __restrict__ psi, double* __restrict__ rphi,
+                double* __restrict__ nx, double* __restrict__ ny, double* __restrict__ nz,
+                Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
+    const double gamma = 1.0 * S->gumax;
+    const double dxi = g.dxi, ddxi = g.ddxi, eps = g.eps;
+    const double uc = u[q] - m0, up = u[idx3(ip, j, k, n)] - m0;
+    const double vc = v[q] - m1, vp = v[idx3(i, jp, k, n)] - m1;
+    const double wc = w[q] - m2, wp = w[idx3(i, j, kp, n)] - m2;
+    const double pc = phi[q];
+    const double pxp = phi[idx3(ip, j, k, n)], pxm = phi[idx3(im, j, k, n)];
+    const double pyp = phi[idx3(i, jp, k, n)], pym = phi[idx3(i, jm, k, n)];
+    const double pzp = phi[idx3(i, j, kp, n)], pzm = phi[idx3(i, j, km, n)];
+    double r = -(up * 0.5 * (pxp + pc) - uc * 0.5 * (pc + pxm)) * dxi
+               - (vp * 0.5 * (pyp + pc) - vc * 0.5 * (pc + pym)) * dxi
+               - (wp * 0.5 * (pzp + pc) - wc * 0.5 * (pc + pzm)) * dxi
+               + gamma * (eps * (pxp - 2.0 * pc + pxm) * ddxi +
+                          eps * (pyp - 2.0 * pc + pym) * ddxi +
+                          eps * (pzp - 2.0 * pc + pzm) * ddxi);
+    rphi[q] = r;
+    nx[q] = (psi[idx3(ip, j, k, n)] - psi[idx3(im, j, k, n)]);
+    ny[q] = (psi[idx3(i, jp, k, n)] - psi[idx3(i, jm, k, n)]);
+    nz[q] = (psi[idx3(i, j, kp, n)] - psi[idx3(i, j, km, n)]);
+}
+
+// P3, main.f90:582-593
+__global__ __launch_bounds__(256)
+void k_acdi_normalize(double* __restrict__ nx, double* __restrict__ ny, double* __restrict__ nz,
+                      Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double a = nx[q], b = ny[q], c = nz[q];
+    const double normag = 1.0 / (sqrt(a * a + b * b + c * c) + g.enm);
+    nx[q] = a * normag;
+    ny[q] = b * normag;
+    nz[q] = c * normag;
+}
+
+// P4 + P5, main.f90:596-630: sharpening flux and AB2 of phi (in place)
+__global__ __launch_bounds__(256)
+void k_acdi_update(double* __restrict__ phi, const double* __restrict__ rphi_in,
+                   double* __restrict__ rphi_o, const double* __restrict__ T,
+                   const double* __restrict__ nx, const double* __restrict__ ny,
+                   const double* __restrict__ nz, Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double gamma = 1.0 * S->gumax;
+    const double alpha = S->alpha, beta = S->beta, dxi = g.dxi;
+    const double Txp = T[idx3(ip, j, k, n)], Txm = T[idx3(im, j, k, n)];
+    const double Typ = T[idx3(i, jp, k, n)], Tym = T[idx3(i, jm, k, n)];
+    const double Tzp = T[idx3(i, j, kp, n)], Tzm = T[idx3(i, j, km, n)];
+    double r = rphi_in[q] -
+               gamma * ((0.25 * (1.0 - Txp * Txp) * nx[idx3(ip, j, k, n)] -
+                         0.25 * (1.0 - Txm * Txm) * nx[idx3(im, j, k, n)]) * 0.5 * dxi +
+                        (0.25 * (1.0 - Typ * Typ) * ny[idx3(i, jp, k, n)] -
+                         0.25 * (1.0 - Tym * Tym) * ny[idx3(i, jm, k, n)]) * 0.5 * dxi +
+                        (0.25 * (1.0 - Tzp * Tzp) * nz[idx3(i, j, kp, n)] -
+                         0.25 * (1.0 - Tzm * Tzm) * nz[idx3(i, j, km, n)]) * 0.5 * dxi);
+    phi[q] = phi[q] + g.dt * (alpha * r - beta * rphi_o[q]);
+    rphi_o[q] = r;
+}
+
+int launch_acdi(mhit_ctx* c) {
+    CellLaunch L(c->n);
+    k_acdi_psi<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->psi, c->tanh_psi, c->g, c->S);
+    k_acdi_rhs<<<L.grid, L.block, 0, c->stream>>>(c->u, c->v, c->w, c->phi, c->psi, c->rhsphi,
+                                                  c->normx, c->normy, c->normz, c->g, c->S);
+    k_acdi_normalize<<<L.grid, L.block, 0, c->stream>>>(c->normx, c->normy, c->normz, c->g, c->S);
+    k_acdi_update<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->rhsphi, c->rhsphi_o, c->tanh_psi,
+                                                     c->normx, c->normy, c->normz, c->g, c->S);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+// ------------------------------------------------------------------ CSF
+// S1, main.f90:721-743 (phi already advanced)
+__global__ __launch_bounds__(256)
+void k_csf(const double* __restrict__ phi, double* __restrict__ fx, double* __restrict__ fy,
+           double* __restrict__ fz, Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double ph = phi[q];
+    const double pxp = phi[idx3(ip, j, k, n)], pxm = phi[idx3(im, j, k, n)];
+    const double pyp = phi[idx3(i, jp, k, n)], pym = phi[idx3(i, jm, k, n)];
+    const double pzp = phi[idx3(i, j, kp, n)], pzm = phi[idx3(i, j, km, n)];
+    const double chempot = ph * (1.0 - ph) * (1.0 - 2.0 * ph) * g.epsi -
+                           g.eps * (pxp + pxm + pyp + pym + pzp + pzm - 6.0 * ph) * g.ddxi;
+    fx[q] = 6.0 * g.sigma * chempot * 0.5 * (pxp - pxm) * g.dxi;
+    fy[q] = 6.0 * g.sigma * chempot * 0.5 * (pyp - pym) * g.dxi;
+    fz[q] = 6.0 * g.sigma * chempot * 0.5 * (pzp - pzm) * g.dxi;
+}
+
+// ------------------------------------------------------------------- NS
+// N1 (main.f90:658-717) fused with the face force + AB2 of S2
+// (main.f90:760-780) or with N2 (783-796).  Writes u* to us,vs,ws.
+template <bool PHI>
+__global__ __launch_bounds__(256)
+void k_ns(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+          const double* __restrict__ fx, const double* __restrict__ fy, const double* __restrict__ fz,
+          double* __restrict__ ru_o, double* __restrict__ rv_o, double* __restrict__ rw_o,
+          double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
+          const double* __restrict__ mysin, const double* __restrict__ mycos,
+          Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    MHIT_CELL_INDEX(g);
+    const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
+    const double alpha = S->alpha, beta = S->beta;
+    const double dxi = g.dxi, ddxi = g.ddxi, mu = g.mu, rhoi = g.rhoi;
+#define U(a, b, c) (u[idx3(a, b, c, n)] - m0)
+#define V(a, b, c) (v[idx3(a, b, c, n)] - m1)
+#define W(a, b, c) (w[idx3(a, b, c, n)] - m2)
+    const double u0 = U(i, j, k), v0 = V(i, j, k), w0 = W(i, j, k);
+    const double uxp = U(ip, j, k), uxm = U(im, j, k), uyp = U(i, jp, k), uym = U(i, jm, k);
+    const double uzp = U(i, j, kp), uzm = U(i, j, km);
+    const double vxp = V(ip, j, k), vxm = V(im, j, k), vyp = V(i, jp, k), vym = V(i, jm, k);
+    const double vzp = V(i, j, kp), vzm = V(i, j, km);
+    const double wxp = W(ip, j, k), wxm = W(im, j, k), wyp = W(i, jp, k), wym = W(i, jm, k);
+    const double wzp = W(i, j, kp), wzm = W(i, j, km);
+    double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+    double h12 = (uyp + u0) * (vyp + V(im, jp, k)) - (u0 + uym) * (v0 + vxm);
+    double h13 = (uzp + u0) * (wzp + W(im, j, kp)) - (u0 + uzm) * (w0 + wxm);
+    double h21 = (uxp + U(ip, jm, k)) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+    double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+    double h23 = (wzp + W(i, jm, kp)) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+    double h31 = (wxp + w0) * (uxp + U(ip, j, km)) - (w0 + wxm) * (u0 + uzm);
+    double h32 = (vyp + V(i, jp, km)) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+    double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+#undef U
+#undef V
+#undef W
+    h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
+    h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
+    h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
+    double ru = -(h11 + h12 + h13);
+    double rv = -(h21 + h22 + h23);
+    double rw = -(h31 + h32 + h33);
+    h11 = mu * (uxp - 2.0 * u0 + uxm) * ddxi;
+    h12 = mu * (uyp - 2.0 * u0 + uym) * ddxi;
+    h13 = mu * (uzp - 2.0 * u0 + uzm) * ddxi;
+    h21 = mu * (vxp - 2.0 * v0 + vxm) * ddxi;
+    h22 = mu * (vyp - 2.0 * v0 + vym) * ddxi;
+    h23 = mu * (vzp - 2.0 * v0 + vzm) * ddxi;
+    h31 = mu * (wxp - 2.0 * w0 + wxm) * ddxi;
+    h32 = mu * (wyp - 2.0 * w0 + wym) * ddxi;
+    h33 = mu * (wzp - 2.0 * w0 + wzm) * ddxi;
+    ru = ru + (h11 + h12 + h13) * rhoi;
+    rv = rv + (h21 + h22 + h23) * rhoi;
+    rw = rw + (h31 + h32 + h33) * rhoi;
+    // ABC forcing: kg = k+2, jg = j+2 in Fortran indexing -> table k+1, j+1
+    ru = ru + g.f3 * mysin[k + 1] + g.f2 * mycos[j + 1];
+    rv = rv + g.f1 * mysin[i] + g.f3 * mycos[k + 1];
+    rw = rw + g.f2 * mysin[j + 1] + g.f1 * mycos[i];
+    if (PHI) {
+        ru = ru + 0.5 * (fx[idx3(im, j, k, n)] + fx[q]) * rhoi;
+        rv = rv + 0.5 * (fy[idx3(i, jm, k, n)] + fy[q]) * rhoi;
+        rw = rw + 0.5 * (fz[idx3(i, j, km, n)] + fz[q]) * rhoi;
+    }
+    const double dt = g.dt;
+    us[q] = u0 + dt * (alpha * ru - beta * ru_o[q]);
+    vs[q] = v0 + dt * (alpha * rv - beta * rv_o[q]);
+    ws[q] = w0 + dt * (alpha * rw - beta * rw_o[q]);
+    ru_o[q] = ru;
+    rv_o[q] = rv;
+    rw_o[q] = rw;
+}
+
+int launch_ns(mhit_ctx* c) {
+    CellLaunch L(c->n);
+    if (c->P.phiflag) {
+        k_csf<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->fxst, c->fyst, c->fzst, c->g, c->S);
+        k_ns<true><<<L.grid, L.block, 0, c->stream>>>(
+            c->u, c->v, c->w, c->fxst, c->fyst, c->fzst, c->rhsu_o, c->rhsv_o, c->rhsw_o,
+            c->us, c->vs, c->ws, c->mysin, c->mycos, c->g, c->S);
+    } else {
+        k_ns<false><<<L.grid, L.block, 0, c->stream>>>(
+            c->u, c->v, c->w, nullptr, nullptr, nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o,
+            c->us, c->vs, c->ws, c->mysin, c->mycos, c->g, c->S);
+    }
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+// ----------------------------------------------------------- correction
+// C, main.f90:942-962: u = u* - dt/rho (p - p(-1)) dxi with the pressure
+// promoted to FP64, plus per-block sums and maxima of the corrected u,v,w.
+// The grid covers (x, y) once and loops over z planes with stride
+// gridDim.z, so the number of partials is fixed and the reduction order is
+// deterministic.
+constexpr int kCorrZ = 16;
+
+template <typename T>
+__global__ __launch_bounds__(256)
+void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
+               const double* __restrict__ ws, const T* __restrict__ p,
+               double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
+               double* __restrict__ partials, Grid g, const DevScalars* __restrict__ S) {
+    __shared__ double red[6][4];
+    const int n = g.n;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const bool live = !S->halted && i < n && j < n;
+    const double dtr = g.dt / g.rho;
+    double su = 0.0, sv = 0.0, sw = 0.0;
+    double mu_ = -INFINITY, mv_ = -INFINITY, mw_ = -INFINITY;
+    if (live) {
+        const int im = wrap_m(i, n), jm = wrap_m(j, n);
+        for (int k = blockIdx.z; k < n; k += gridDim.z) {
+            const int km = wrap_m(k, n);
+            const size_t q = idx3(i, j, k, n);
+            const double pc = (double)p[q];
+            const double uu = us[q] - dtr * (pc - (double)p[idx3(im, j, k, n)]) * g.dxi;
+            const double vv = vs[q] - dtr * (pc - (double)p[idx3(i, jm, k, n)]) * g.dxi;
+            const double ww = ws[q] - dtr * (pc - (double)p[idx3(i, j, km, n)]) * g.dxi;
+            u[q] = uu; v[q] = vv; w[q] = ww;
+            su += uu; sv += vv; sw += ww;
+            mu_ = fmax(mu_, uu); mv_ = fmax(mv_, vv); mw_ = fmax(mw_, ww);
+        }
+    }
+    su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
+    mu_ = wave_max(mu_); mv_ = wave_max(mv_); mw_ = wave_max(mw_);
+    const int lane = threadIdx.x & 63;        // blockDim.x == 64: one wave per y row
+    const int wv = threadIdx.y;
+    if (lane == 0) {
+        red[0][wv] = su; red[1][wv] = sv; red[2][wv] = sw;
+        red[3][wv] = mu_; red[4][wv] = mv_; red[5][wv] = mw_;
+    }
+    __syncthreads();
+    if (threadIdx.y == 0 && threadIdx.x < 6) {
+        const int f = threadIdx.x;
+        double r = red[f][0];
+        for (int t = 1; t < 4; t++) r = (f < 3) ? r + red[f][t] : fmax(r, red[f][t]);
+        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[b * 6 + f] = r;
+    }
+}
+
+int launch_correct(mhit_ctx* c) {
+    const int n = c->n;
+    dim3 grid(div_up(n, 64), div_up(n, 4), kCorrZ < n ? kCorrZ : n), block(64, 4, 1);
+    if (c->fp32)
+        k_correct<float><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const float*)c->pres,
+                                                       c->u, c->v, c->w, c->partials, c->g, c->S);
+    else
+        k_correct<double><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const double*)c->pres,
+                                                        c->u, c->v, c->w, c->partials, c->g, c->S);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+int correct_block_count(int n) {
+    return div_up(n, 64) * div_up(n, 4) * (kCorrZ < n ? kCorrZ : n);
+}
+
+// Finalize: means (main.f90:964-974), signed max after the mean removal
+// (998-1002, max(u - m) = max(u) - m as rounding is monotone), CFL
+// (1003-1006), AB2 switch (801-802).  One block, fixed reduction order.
+__global__ __launch_bounds__(1024)
+void k_finalize(const double* __restrict__ partials, int nblocks, Grid g, int cfl_abort,
+                DevScalars* __restrict__ S) {
+    __shared__ double red[6][1024];
+    const int t = threadIdx.x;
+    if (S->halted) return;
+    double a[6] = {0.0, 0.0, 0.0, -INFINITY, -INFINITY, -INFINITY};
+    for (int b = t; b < nblocks; b += 1024) {
+        for (int f = 0; f < 3; f++) a[f] += partials[(size_t)b * 6 + f];
+        for (int f = 3; f < 6; f++) a[f] = fmax(a[f], partials[(size_t)b * 6 + f]);
+    }
+    for (int f = 0; f < 6; f++) red[f][t] = a[f];
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s) {
+            for (int f = 0; f < 3; f++) red[f][t] += red[f][t + s];
+            for (int f = 3; f < 6; f++) red[f][t] = fmax(red[f][t], red[f][t + s]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double nn = (double)g.n;
+        double m[3], mx[3];
+        for (int f = 0; f < 3; f++) {
+            m[f] = red[f][0] / nn / nn / nn;
+            mx[f] = red[f + 3][0] - m[f];
+        }
+        const double umax = fmax(mx[2], fmax(mx[0], mx[1]));
+        S->mean[0] = m[0]; S->mean[1] = m[1]; S->mean[2] = m[2];
+        S->gumax = umax;
+        S->cou = umax * g.dt * g.dxi;
+        S->alpha = 1.5;
+        S->beta = 0.5;
+        S->step += 1;
+        if (cfl_abort && S->cou > 7.0) S->halted = 1;
+    }
+}
+
+int launch_finalize(mhit_ctx* c) {
+    k_finalize<<<1, 1024, 0, c->stream>>>(c->partials, c->npart_blocks, c->g, c->P.cfl_abort, c->S);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+// ---------------------------------------------------------- diagnostics
+// KE, dissipation 2 nu <S_ij S_ij> with central differences
+// (post/compute_diss/main.f90:72-117, dx = lx/n, nu = mu/rho), max |D.u|
+// with the divergence stencil of main.f90:841-843, phi mass and range.
+__global__ __launch_bounds__(256)
+void k_diag(const double* __restrict__ u, const double* __restrict__ v,
+            const double* __restrict__ w, const double* __restrict__ phi,
+            double* __restrict__ partials, Grid g, const DevScalars* __restrict__ S) {
+    constexpr int NF = 8;
+    __shared__ double red[NF][4];
+    const int n = g.n;
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    double acc[NF] = {0.0, 0.0, 0.0, 0.0, 0.0, INFINITY, -INFINITY, 0.0};
+    if (i < n && j < n) {
+        const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
+        const int ip = wrap_p(i, n), im = wrap_m(i, n), jp = wrap_p(j, n), jm = wrap_m(j, n);
+        const double h = 0.5 * g.dxi;
+        for (int k = blockIdx.z; k < n; k += gridDim.z) {
+            const int kp = wrap_p(k, n), km = wrap_m(k, n);
+            const size_t q = idx3(i, j, k, n);
+#define U(a, b, c) (u[idx3(a, b, c, n)] - m0)
+#define V(a, b, c) (v[idx3(a, b, c, n)] - m1)
+#define W(a, b, c) (w[idx3(a, b, c, n)] - m2)
+            const double uu = U(i, j, k), vv = V(i, j, k), ww = W(i, j, k);
+            if (!isfinite(uu) || !isfinite(vv) || !isfinite(ww)) acc[7] += 1.0;
+            acc[0] += uu * uu + vv * vv + ww * ww;
+            const double dudx = (U(ip, j, k) - U(im, j, k)) * h, dudy = (U(i, jp, k) - U(i, jm, k)) * h;
+            const double dudz = (U(i, j, kp) - U(i, j, km)) * h;
+            const double dvdx = (V(ip, j, k) - V(im, j, k)) * h, dvdy = (V(i, jp, k) - V(i, jm, k)) * h;
+            const double dvdz = (V(i, j, kp) - V(i, j, km)) * h;
+            const double dwdx = (W(ip, j, k) - W(im, j, k)) * h, dwdy = (W(i, jp, k) - W(i, jm, k)) * h;
+            const double dwdz = (W(i, j, kp) - W(i, j, km)) * h;
+            const double S12 = 0.5 * (dudy + dvdx), S13 = 0.5 * (dudz + dwdx), S23 = 0.5 * (dvdz + dwdy);
+            acc[1] += dudx * dudx + dvdy * dvdy + dwdz * dwdz + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+            const double dv = ((U(ip, j, k) - uu) + (V(i, jp, k) - vv) + (W(i, j, kp) - ww)) * g.dxi;
+#undef U
+#undef V
+#undef W
+            acc[2] = fmax(acc[2], fabs(dv));
+            if (phi) {
+                const double ph = phi[q];
+                if (!isfinite(ph)) acc[7] += 1.0;
+                acc[3] += ph;
+                acc[5] = fmin(acc[5], ph);
+                acc[6] = fmax(acc[6], ph);
+            }
+        }
+    }
+    for (int f = 0; f < NF; f++) {
+        double x = acc[f];
+        for (int off = 32; off > 0; off >>= 1) {
+            const double y = __shfl_xor(x, off, 64);
+            x = (f == 2 || f == 6) ? fmax(x, y) : (f == 5 ? fmin(x, y) : x + y);
+        }
+        acc[f] = x;
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int f = 0; f < NF; f++) red[f][threadIdx.y] = acc[f];
+    __syncthreads();
+    if (threadIdx.y == 0 && threadIdx.x < NF) {
+        const int f = threadIdx.x;
+        double r = red[f][0];
+        for (int t = 1; t < 4; t++)
+            r = (f == 2 || f == 6) ? fmax(r, red[f][t]) : (f == 5 ? fmin(r, red[f][t]) : r + red[f][t]);
+        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partials[b * NF + f] = r;
+    }
+}
+
+int launch_diag(mhit_ctx* c, double* out) {
+    const int n = c->n;
+    dim3 grid(div_up(n, 64), div_up(n, 4), kCorrZ < n ? kCorrZ : n), block(64, 4, 1);
+    k_diag<<<grid, block, 0, c->stream>>>(c->u, c->v, c->w, c->P.phiflag ? c->phi : nullptr,
+                                          c->dpartials, c->g, c->S);
+    MHIT_HIP_TRY(hipGetLastError());
+    const int nb = grid.x * grid.y * grid.z;
+    std::vector<double> h((size_t)nb * 8);
+    MHIT_HIP_TRY(hipMemcpyAsync(h.data(), c->dpartials, h.size() * sizeof(double),
+                                hipMemcpyDeviceToHost, c->stream));
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    double r[8] = {0.0, 0.0, 0.0, 0.0, 0.0, INFINITY, -INFINITY, 0.0};
+    for (int b = 0; b < nb; b++)
+        for (int f = 0; f < 8; f++) {
+            const double x = h[(size_t)b * 8 + f];
+            r[f] = (f == 2 || f == 6) ? std::fmax(r[f], x) : (f == 5 ? std::fmin(r[f], x) : r[f] + x);
+        }
+    for (int f = 0; f < 8; f++) out[f] = r[f];
+    return MHIT_OK;
+}
+
+}  // namespace mhit

@@ -1,0 +1,106 @@
+// tracers.hip -- Lagrangian tracer step (gfx950).
+//
+// One thread per tracer: interpolate u^n at the tracer (particles.f90:78-122),
+// advance with AB2/Euler (main.f90:480-486), wrap into the box
+// (particles.f90:364-380) and keep the interpolated velocity as the
+// previous-step velocity (main.f90:503-508).  At one rank the reference's
+// y/z sort and send-to-self (main.f90:489-498) only permute the tracers,
+// so it has no counterpart here; tracers keep their order and their tag.
+#include "internal.hpp"
+
+namespace mhit {
+
+__device__ __forceinline__ int pmod(long long a, int n) {
+    long long r = a % n;
+    return (int)(r < 0 ? r + n : r);
+}
+
+// Reference interpolation: 1-D linear per component along its own axis.
+// The y/z indices are those of the halo-shifted local array
+// (iy = 1 + floor(y/dx) with index 1 the halo) -> interior (fy-1) mod n.
+__device__ __forceinline__ void interp_reference(const double* __restrict__ u, const double* __restrict__ v,
+                                                 const double* __restrict__ w, const double* m,
+                                                 double xp, double yp, double zp, const Grid& g,
+                                                 double& up, double& vp, double& wp) {
+    const int n = g.n;
+    const double dx = g.dx;
+    const long long fx = (long long)floor(xp / dx), fy = (long long)floor(yp / dx),
+                    fz = (long long)floor(zp / dx);
+    const double tx = xp / dx - (double)fx;
+    const double ty = yp / dx - (double)fy;
+    const double tz = zp / dx - (double)fz;
+    const int i0 = pmod(fx, n), i1 = pmod(fx + 1, n);
+    const int j0 = pmod(fy - 1, n), j1 = pmod(fy, n);
+    const int k0 = pmod(fz - 1, n), k1 = pmod(fz, n);
+    const size_t q = idx3(i0, j0, k0, n);
+    up = (1.0 - tx) * (u[q] - m[0]) + tx * (u[idx3(i1, j0, k0, n)] - m[0]);
+    vp = (1.0 - ty) * (v[q] - m[1]) + ty * (v[idx3(i0, j1, k0, n)] - m[1]);
+    wp = (1.0 - tz) * (w[q] - m[2]) + tz * (w[idx3(i0, j0, k1, n)] - m[2]);
+}
+
+// Trilinear option: each component at its staggered location in the
+// time-stepper frame (u at ((i-1/2)dx,(j+1)dx,(k+1)dx), v at
+// (i dx,(j+1/2)dx,(k+1)dx), w at (i dx,(j+1)dx,(k+1/2)dx)).
+__device__ __forceinline__ double trilin(const double* __restrict__ f, double mean, double xs, double ys,
+                                         double zs, int n) {
+    const double fxl = floor(xs), fyl = floor(ys), fzl = floor(zs);
+    const double tx = xs - fxl, ty = ys - fyl, tz = zs - fzl;
+    const long long ix = (long long)fxl, iy = (long long)fyl, iz = (long long)fzl;
+    const int i0 = pmod(ix, n), i1 = pmod(ix + 1, n);
+    const int j0 = pmod(iy, n), j1 = pmod(iy + 1, n);
+    const int k0 = pmod(iz, n), k1 = pmod(iz + 1, n);
+    const double c00 = (1.0 - tx) * (f[idx3(i0, j0, k0, n)] - mean) + tx * (f[idx3(i1, j0, k0, n)] - mean);
+    const double c10 = (1.0 - tx) * (f[idx3(i0, j1, k0, n)] - mean) + tx * (f[idx3(i1, j1, k0, n)] - mean);
+    const double c01 = (1.0 - tx) * (f[idx3(i0, j0, k1, n)] - mean) + tx * (f[idx3(i1, j0, k1, n)] - mean);
+    const double c11 = (1.0 - tx) * (f[idx3(i0, j1, k1, n)] - mean) + tx * (f[idx3(i1, j1, k1, n)] - mean);
+    const double c0 = (1.0 - ty) * c00 + ty * c10;
+    const double c1 = (1.0 - ty) * c01 + ty * c11;
+    return (1.0 - tz) * c0 + tz * c1;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256)
+void k_tracers(const double* __restrict__ u, const double* __restrict__ v, const double* __restrict__ w,
+               double* __restrict__ px, double* __restrict__ py, double* __restrict__ pz,
+               double* __restrict__ pu1, double* __restrict__ pv1, double* __restrict__ pw1,
+               long long np, Grid g, const DevScalars* __restrict__ S) {
+    if (S->halted) return;
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const double m[3] = {S->mean[0], S->mean[1], S->mean[2]};
+    const double alpha = S->alpha, beta = S->beta, dt = g.dt, lx = g.lx;
+    double x = px[p], y = py[p], z = pz[p];
+    double up, vp, wp;
+    if (MODE == MHIT_INTERP_TRILINEAR) {
+        const double xs = x / g.dx, ys = y / g.dx, zs = z / g.dx;
+        up = trilin(u, m[0], xs + 0.5, ys - 1.0, zs - 1.0, g.n);
+        vp = trilin(v, m[1], xs, ys - 0.5, zs - 1.0, g.n);
+        wp = trilin(w, m[2], xs, ys - 1.0, zs - 0.5, g.n);
+    } else {
+        interp_reference(u, v, w, m, x, y, z, g, up, vp, wp);
+    }
+    x = x + dt * (alpha * up - beta * pu1[p]);
+    y = y + dt * (alpha * vp - beta * pv1[p]);
+    z = z + dt * (alpha * wp - beta * pw1[p]);
+    // FLOOR returns a default integer in the reference
+    x = x - lx * (double)(int)floor(x / lx);
+    y = y - lx * (double)(int)floor(y / lx);
+    z = z - lx * (double)(int)floor(z / lx);
+    px[p] = x; py[p] = y; pz[p] = z;
+    pu1[p] = up; pv1[p] = vp; pw1[p] = wp;
+}
+
+int launch_tracers(mhit_ctx* c) {
+    if (c->np <= 0) return MHIT_OK;
+    const int blocks = div_up(c->np, 256);
+    if (c->P.interp_mode == MHIT_INTERP_TRILINEAR)
+        k_tracers<MHIT_INTERP_TRILINEAR><<<blocks, 256, 0, c->stream>>>(
+            c->u, c->v, c->w, c->px, c->py, c->pz, c->pu1, c->pv1, c->pw1, c->np, c->g, c->S);
+    else
+        k_tracers<MHIT_INTERP_REFERENCE><<<blocks, 256, 0, c->stream>>>(
+            c->u, c->v, c->w, c->px, c->py, c->pz, c->pu1, c->pv1, c->pw1, c->np, c->g, c->S);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+}  // namespace mhit

@@ -1,0 +1,79 @@
+"""GPU FFT / Poisson parity (HIP path through the C-ABI).
+
+Pins: numpy.fft (library routine) for the 3-D R2C/C2R transforms, the
+closed-form single-mode Poisson solution, and the CPU oracle's Poisson
+solve (multi/main.f90:853-912).  Tolerances: FP64 path 1e-12 relative to
+the field's max; FP32 path 2e-6 (transform) / 1e-5 (solve) relative."""
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [8, 12, 16, 24, 32, 48, 64, 96, 128]
+
+
+def _solver(n, prec, **kw):
+    from mhit36_mixedp_amd import Solver
+    return Solver(nx=n, poisson_precision=prec, **kw)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", SIZES)
+def test_r2c_matches_numpy(n, prec):
+    print(f"[fft r2c] n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    x = rng.standard_normal((n, n, n))
+    with _solver(n, prec) as s:
+        X = s.fft_r2c(x)
+    ref = np.fft.rfftn(x)
+    tol = 1e-12 if prec == 0 else 2e-6
+    assert relerr(X, ref) < tol, relerr(X, ref)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", SIZES)
+def test_c2r_matches_numpy(n, prec):
+    print(f"[fft c2r] n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(200 + n)
+    x = rng.standard_normal((n, n, n))
+    X = np.fft.rfftn(x)
+    with _solver(n, prec) as s:
+        y = s.fft_c2r(X)
+    ref = np.fft.irfftn(X, s=(n, n, n), axes=(0, 1, 2)) * n ** 3
+    tol = 1e-12 if prec == 0 else 2e-6
+    assert relerr(y, ref) < tol, relerr(y, ref)
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [16, 32, 96])
+def test_poisson_single_mode(n, prec):
+    """rhs = cos(m.x + 0.3) => p = -rhs/|m|^2 for the continuous k^2."""
+    print(f"[poisson mode] n={n} prec={prec}", flush=True)
+    L = 2 * np.pi
+    xs = np.arange(n) * (L / n)
+    Z, Y, X = np.meshgrid(xs, xs, xs, indexing="ij")
+    m = (3, -2, 5)
+    rhs = np.cos(m[0] * X + m[1] * Y + m[2] * Z + 0.3)
+    with _solver(n, prec) as s:
+        p = s.poisson_solve(rhs)
+    ref = -rhs / sum(a * a for a in m)
+    tol = 1e-13 if prec == 0 else 2e-6
+    assert relerr(p, ref) < tol
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("k2", [0, 1], ids=["k2", "k2mod"])
+@pytest.mark.parametrize("n", [32, 64])
+def test_poisson_vs_oracle(n, prec, k2, oracle_mod):
+    print(f"[poisson oracle] n={n} prec={prec} k2={k2}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    rhs = rng.standard_normal((n, n, n))
+    rhs -= rhs.mean()
+    with _solver(n, prec, k2_mode=k2) as s:
+        p = s.poisson_solve(rhs)
+    o = oracle_mod.Oracle(nx=n, poisson_precision=prec, k2_mode=k2)
+    ref = o.poisson(rhs)
+    tol = 1e-12 if prec == 0 else 1e-5
+    assert relerr(p, ref) < tol, relerr(p, ref)
