@@ -1,0 +1,209 @@
+#!/usr/bin/env python3
+"""MHIT36 single-GPU timestep benchmark on MI355X.
+
+Workload (default): 512^3 multiphase box -- random-phase velocity
+(|k| <= 8, u_rms = 1, seeded), ACDI phase field with 512 seeded drops of
+radius 0.1 (eps = dx), CSF surface tension at We = rho u_rms^2 (2R)/sigma
+= 1, 4,194,304 uniform tracers, AB2 with dt = 1e-4, nu = 0.002.  All inputs
+are generated on the device before the timed region.
+
+Prints ONE JSON line: ms per timestep of the FP32-Poisson form (the
+"mixedp" form of the reference) with the FP64-Poisson form beside it, the
+whole-step HBM rate over the algorithmic bytes, a roofline entry for the
+dominant kernel (HIP events on the library's stream) and the CPU oracle
+timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def step_bytes_fused(n: int, ntr: int, phase: bool, fp32: bool) -> float:
+    """Compulsory HBM bytes per step with every array read and written once
+    per pass: 208/248 B per cell (NS, FP32/FP64 Poisson), 272/312 with the
+    phase field, plus 144 B per tracer."""
+    per_cell = (272.0 if fp32 else 312.0) if phase else (208.0 if fp32 else 248.0)
+    return per_cell * n ** 3 + 144.0 * ntr
+
+
+def make_solver(args, prec):
+    from mhit36_mixedp_amd import Solver
+    sigma = 1.0 * 1.0 ** 2 * (2 * args.radius) / args.we
+    s = Solver(nx=args.n, poisson_precision=prec, phiflag=1, partflag=1, dt=args.dt, mu=args.mu,
+               rho=1.0, sigma=sigma, epsr=1.0, npart=args.ntracers)
+    s.init_random_velocity(args.seed, kmax=8.0, urms=1.0)
+    s.init_drops(args.ndrops, args.radius, args.seed)
+    s.init_tracers_uniform(args.ntracers, args.seed)
+    return s
+
+
+def time_steps(s, warmup, steps):
+    s.step(warmup)
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(steps)           # returns after a stream synchronize
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1e3 / steps
+
+
+def cpu_baseline(args, fields, tracers):
+    """Time the CPU oracle on one step of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    oracle.set_threads(threads)
+    sigma = (2 * args.radius) / args.we
+    o = oracle.Oracle(nx=args.n, poisson_precision=1, phiflag=1, partflag=1, dt=args.dt, mu=args.mu,
+                      rho=1.0, sigma=sigma, epsr=1.0)
+    for f, a in fields.items():
+        o.set_field(f, a)
+    o.set_tracers(*tracers)
+    t0 = time.perf_counter()
+    rc = o.step(args.cpu_steps)
+    t1 = time.perf_counter()
+    o.close()
+    if rc != 0:
+        raise RuntimeError(f"oracle step failed: {rc}")
+    return (t1 - t0) * 1e3 / args.cpu_steps, threads
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--ndrops", type=int, default=512)
+    ap.add_argument("--radius", type=float, default=0.1)
+    ap.add_argument("--ntracers", type=int, default=4194304)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--mu", type=float, default=0.002)
+    ap.add_argument("--we", type=float, default=1.0)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-Poisson form")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs one box on one GPU; --gpus {args.gpus} is not supported "
+              "(independent boxes shard trivially across GPUs; see the sharding notes).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from mhit36_mixedp_amd import FIELD_PHI, FIELD_U, FIELD_V, FIELD_W, version
+    log(f"[bench] {version()}  n={args.n} drops={args.ndrops} tracers={args.ntracers}")
+
+    # --- FP32-Poisson form (the measured value) --------------------------------
+    s = make_solver(args, prec=1)
+    fields = tracers = None
+    if not args.no_cpu_baseline:
+        fields = {f: s.download_field(f) for f in (FIELD_U, FIELD_V, FIELD_W, FIELD_PHI)}
+        t = s.download_tracers()
+        tracers = (t["x"], t["y"], t["z"])
+    ms32 = time_steps(s, args.warmup, args.steps)
+    log(f"[bench] FP32 Poisson: {ms32:.3f} ms/step")
+    s.profile_enable(True)
+    s.step(args.profile_steps)
+    prof32 = s.profile()
+    s.profile_enable(False)
+    d = s.diag()
+    log(f"[bench] diag after {d['step']} steps: ke={d['ke']:.6e} gumax={d['gumax']:.4f} "
+        f"cou={d['cou']:.3f} maxdiv={d['maxdiv']:.3e} phi_mass={d['phi_mass']:.6e} nan={d['nan_count']}")
+    if d["nan_count"]:
+        raise RuntimeError("non-finite values in the state")
+    s.close()
+    del s
+
+    # --- FP64-Poisson form ------------------------------------------------------
+    ms64 = None
+    prof64 = None
+    if not args.no_fp64:
+        s = make_solver(args, prec=0)
+        ms64 = time_steps(s, args.warmup, args.steps)
+        log(f"[bench] FP64 Poisson: {ms64:.3f} ms/step")
+        s.profile_enable(True)
+        s.step(args.profile_steps)
+        prof64 = s.profile()
+        s.close()
+        del s
+
+    # --- per-stage table and roofline of the dominant kernel ----------------------
+    log(f"[bench] per-stage (FP32 Poisson, {args.profile_steps} profiled steps):")
+    for r in prof32:
+        avg = r["ms"] / r["calls"]
+        gbps = r["bytes_per_call"] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+        log(f"   {r['stage']:<14} {avg:8.3f} ms  {r['bytes_per_call'] / 1e9:7.3f} GB  {gbps:8.1f} GB/s")
+    dom = max(prof32, key=lambda r: r["ms"])
+    dom_avg_ms = dom["ms"] / dom["calls"]
+    dom_gbps = dom["bytes_per_call"] / (dom_avg_ms * 1e-3) / 1e9
+
+    step_bytes32 = step_bytes_fused(args.n, args.ntracers, True, True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle on the same workload ...")
+        cpu_ms, threads = cpu_baseline(args, fields, tracers)
+        log(f"[bench] CPU oracle: {cpu_ms:.1f} ms/step on {threads} threads")
+        cpu = {"value": round(cpu_ms, 3), "unit": "ms/timestep", "cores": threads, "kind": "port",
+               "sample": f"{args.cpu_steps} timestep(s) of the same {args.n}^3 multiphase workload "
+                         f"({args.ndrops} drops, {args.ntracers} tracers, FP32 Poisson) on the CPU "
+                         "FP64 restatement (oracle/), OpenMP"}
+
+    out = {
+        "metric": "ms/timestep & HBM GB/s at 512^3 multiphase, FP32 vs FP64 Poisson, 1 MI355X",
+        "value": round(ms32, 4),
+        "unit": "ms/timestep",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms32, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 state, f32 Poisson",
+        "data": "synthetic (seeded on device)",
+        "config": {
+            "workload": f"{args.n}^3 multiphase: {args.ndrops} drops R={args.radius}, "
+                        f"{args.ntracers} tracers, dt={args.dt}, nu={args.mu}, We={args.we}",
+            "n": args.n, "ndrops": args.ndrops, "radius": args.radius, "ntracers": args.ntracers,
+            "poisson": "fp32", "parallelism": "single GPU",
+        },
+        "hbm_GBps": round(step_bytes32 / (ms32 * 1e-3) / 1e9, 1),
+        "step_bytes": step_bytes32,
+        "fp64_poisson_ms_per_step": round(ms64, 4) if ms64 else None,
+        "fp64_poisson_hbm_GBps": round(step_bytes_fused(args.n, args.ntracers, True, False)
+                                       / (ms64 * 1e-3) / 1e9, 1) if ms64 else None,
+        "fp32_over_fp64_rate": round(ms64 / ms32, 4) if ms64 else None,
+        "roofline": {
+            "kernel": dom["stage"],
+            "bound": "hbm",
+            "achieved": round(dom_gbps, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(dom_gbps / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "avg_ms": round(dom_avg_ms, 4),
+            "bytes_per_launch": dom["bytes_per_call"],
+        },
+        "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -186,6 +186,23 @@ int mhit_fft_r2c(mhit_ctx* ctx, const double* in, double* out);
 /* Inverse unnormalised 3-D C2R FFT (e^{+i}) of the same layout.           */
 int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
 
+/* ---- per-stage timing ------------------------------------------------ */
+/* With profiling on, mhit_step records a HIP event after every stage of
+ * the step on the context's stream and accumulates the elapsed time per
+ * stage (this adds one event per stage; leave it off for timing whole
+ * steps).  bytes_per_call is the stage's algorithmic HBM traffic: every
+ * array it reads or writes counted once.                                 */
+#define MHIT_PROF_MAX 24
+typedef struct mhit_profile {
+    int32_t nstages;
+    char    name[MHIT_PROF_MAX][32];
+    int64_t calls[MHIT_PROF_MAX];
+    double  ms[MHIT_PROF_MAX];              /* accumulated                */
+    double  bytes_per_call[MHIT_PROF_MAX];
+} mhit_profile;
+int mhit_profile_enable(mhit_ctx* ctx, int on);   /* also resets counters */
+int mhit_profile_read(mhit_ctx* ctx, mhit_profile* out);
+
 /* Library version string.                                                 */
 const char* mhit_version(void);
 

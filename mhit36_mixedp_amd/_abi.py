@@ -87,6 +87,32 @@ class Diag(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+PROF_MAX = 24
+
+
+class Profile(ctypes.Structure):
+    """mhit_profile: per-stage event timing of mhit_step."""
+
+    _fields_ = [
+        ("nstages", ctypes.c_int32),
+        ("name", (ctypes.c_char * 32) * PROF_MAX),
+        ("calls", ctypes.c_int64 * PROF_MAX),
+        ("ms", ctypes.c_double * PROF_MAX),
+        ("bytes_per_call", ctypes.c_double * PROF_MAX),
+    ]
+
+    def as_rows(self):
+        rows = []
+        for s in range(self.nstages):
+            rows.append({
+                "stage": bytes(self.name[s]).split(b"\0", 1)[0].decode(),
+                "calls": int(self.calls[s]),
+                "ms": float(self.ms[s]),
+                "bytes_per_call": float(self.bytes_per_call[s]),
+            })
+        return rows
+
+
 def default_params(**overrides) -> Params:
     """Reference defaults: multi/input.inp values with nx = 256
     (multi/module.f90:2), FP32 Poisson, continuous k^2."""
@@ -135,6 +161,8 @@ EXPORTS = [
     ("mhit_poisson_solve", ctypes.c_int, [_vp, _dp, _dp]),
     ("mhit_fft_r2c", ctypes.c_int, [_vp, _dp, _dp]),
     ("mhit_fft_c2r", ctypes.c_int, [_vp, _dp, _dp]),
+    ("mhit_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("mhit_profile_read", ctypes.c_int, [_vp, _P(Profile)]),
     ("mhit_version", ctypes.c_char_p, []),
 ]
 

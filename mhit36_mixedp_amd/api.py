@@ -134,6 +134,15 @@ class Solver:
         _check(self.lib.mhit_diag_compute(self._h, ctypes.byref(d)), "mhit_diag_compute")
         return d.as_dict()
 
+    def profile_enable(self, on: bool = True):
+        _check(self.lib.mhit_profile_enable(self._h, int(on)), "mhit_profile_enable")
+
+    def profile(self) -> list:
+        """Per-stage event timing accumulated since profile_enable()."""
+        p = _abi.Profile()
+        _check(self.lib.mhit_profile_read(self._h, ctypes.byref(p)), "mhit_profile_read")
+        return [r for r in p.as_rows() if r["calls"] > 0]
+
     # initial conditions -------------------------------------------------------
     def init_taylor_green(self):
         _check(self.lib.mhit_init_taylor_green(self._h), "mhit_init_taylor_green")

@@ -78,15 +78,61 @@ static double* state_field(mhit_ctx* c, int fieldn) {
     }
 }
 
+const char* stage_name(int s) {
+    static const char* names[ST_COUNT] = {
+        "tracers", "acdi_psi", "acdi_rhs", "acdi_normals", "acdi_update", "acdi", "csf", "ns_ab2",
+        "fft_x_fwd_div", "fft_y_fwd", "fft_z_solve", "fft_y_inv", "fft_x_inv", "correct",
+        "finalize"};
+    return (s >= 0 && s < ST_COUNT) ? names[s] : "?";
+}
+
+// Algorithmic HBM bytes of one call of stage s: every array the stage
+// reads or writes counted once (stencil neighbours are free).
+double stage_bytes(const mhit_ctx* c, int s) {
+    const double N3 = (double)c->N3, n = c->n;
+    const double cs = c->fp32 ? 8.0 : 16.0, rs = c->fp32 ? 4.0 : 8.0;
+    const double spec = (n / 2) * n * n * cs + n * n * cs;   // half spectrum + Nyquist plane
+    switch (s) {
+        case ST_TRACERS: return 144.0 * (double)c->np;       // pos r/w, u_old r/w, 6 gathers
+        case ST_ACDI_PSI: return 24.0 * N3;                  // r phi; w psi, tanh
+        case ST_ACDI_RHS: return 72.0 * N3;                  // r u,v,w,phi,psi; w rphi, n x3
+        case ST_ACDI_NORM: return 48.0 * N3;                 // r/w n x3
+        case ST_ACDI_UPDATE: return 72.0 * N3;               // r phi,rphi,rphi_o,T,n x3; w phi,rphi_o
+        case ST_ACDI: return 56.0 * N3;                      // r phi,u,v,w,rphi_o; w phi',rphi_o
+        case ST_CSF: return 32.0 * N3;                       // r phi; w f x3
+        case ST_NS: return (c->P.phiflag ? 120.0 : 96.0) * N3; // r u,v,w,rhs_o[,f x3]; w u*,rhs_o
+        case ST_FFT_X_FWD: return 24.0 * N3 + spec;          // r u*,v*,w*; w spectrum
+        case ST_FFT_Y_FWD: case ST_FFT_Z: case ST_FFT_Y_INV: return 2.0 * spec;
+        case ST_FFT_X_INV: return spec + rs * N3;            // r spectrum; w p
+        case ST_CORRECT: return (48.0 + rs) * N3;            // r u*, p; w u
+        case ST_FINALIZE: return 48.0 * c->npart_blocks;
+        default: return 0.0;
+    }
+}
+
 // one timestep, main.f90:456-1007
 static int enqueue_step(mhit_ctx* c) {
     int rc;
+    if (c->prof) {
+        c->prof_n = 0;
+        prof_mark(c, -1);
+    }
     if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
     if (c->P.phiflag && (rc = launch_acdi(c))) return rc;                    // 521-637
     if ((rc = launch_ns(c))) return rc;                                      // 658-796
     if ((rc = poisson_project(c))) return rc;                                // 833-923
     if ((rc = launch_correct(c))) return rc;                                 // 942-962
     if ((rc = launch_finalize(c))) return rc;                                // 964-1007
+    if (c->prof) {
+        MHIT_HIP_TRY(hipEventSynchronize(c->prof_ev[c->prof_n - 1]));
+        for (int e = 1; e < c->prof_n; e++) {
+            float ms = 0.0f;
+            MHIT_HIP_TRY(hipEventElapsedTime(&ms, c->prof_ev[e - 1], c->prof_ev[e]));
+            const int s = c->prof_stage[e];
+            c->prof_ms[s] += ms;
+            c->prof_calls[s] += 1;
+        }
+    }
     return MHIT_OK;
 }
 
@@ -260,6 +306,8 @@ int mhit_destroy(mhit_ctx* c) {
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
                     c->pu1, c->pv1, c->pw1};
     for (void* p : ptrs) dfree(p);
+    for (hipEvent_t& e : c->prof_ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MHIT_OK;
@@ -379,6 +427,29 @@ int mhit_set_ab2_state(mhit_ctx* c, double alpha, double beta, double gumax) {
 int mhit_sync(mhit_ctx* c) {
     if (!c) return MHIT_ERR_ARG;
     MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+int mhit_profile_enable(mhit_ctx* c, int on) {
+    if (!c) return MHIT_ERR_ARG;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (on && !c->prof_ev[0])
+        for (hipEvent_t& e : c->prof_ev) MHIT_HIP_TRY(hipEventCreate(&e));
+    c->prof = on != 0;
+    for (int s = 0; s < MHIT_PROF_MAX; s++) { c->prof_ms[s] = 0.0; c->prof_calls[s] = 0; }
+    return MHIT_OK;
+}
+
+int mhit_profile_read(mhit_ctx* c, mhit_profile* out) {
+    if (!c || !out) return MHIT_ERR_ARG;
+    std::memset(out, 0, sizeof(*out));
+    out->nstages = ST_COUNT;
+    for (int s = 0; s < ST_COUNT; s++) {
+        std::strncpy(out->name[s], stage_name(s), 31);
+        out->calls[s] = c->prof_calls[s];
+        out->ms[s] = c->prof_ms[s];
+        out->bytes_per_call[s] = stage_bytes(c, s);
+    }
     return MHIT_OK;
 }
 

@@ -44,9 +44,33 @@ struct mhit_ctx {
     double *pu1 = nullptr, *pv1 = nullptr, *pw1 = nullptr;
     // host copy of scalars after the last sync
     mhit::DevScalars S_host{};
+    // per-stage event profiling (mhit_profile_enable)
+    bool prof = false;
+    hipEvent_t prof_ev[40] = {};
+    int prof_stage[40] = {};
+    int prof_n = 0;
+    double prof_ms[MHIT_PROF_MAX] = {};
+    long long prof_calls[MHIT_PROF_MAX] = {};
 };
 
 namespace mhit {
+
+// Stages of one timestep, in launch order.
+enum Stage {
+    ST_TRACERS, ST_ACDI_PSI, ST_ACDI_RHS, ST_ACDI_NORM, ST_ACDI_UPDATE, ST_ACDI, ST_CSF, ST_NS,
+    ST_FFT_X_FWD, ST_FFT_Y_FWD, ST_FFT_Z, ST_FFT_Y_INV, ST_FFT_X_INV, ST_CORRECT, ST_FINALIZE,
+    ST_COUNT
+};
+const char* stage_name(int s);
+double stage_bytes(const mhit_ctx* c, int s);
+
+// Record the end of stage s on the context stream (profiling only).
+inline void prof_mark(mhit_ctx* c, int s) {
+    if (!c->prof || c->prof_n >= 40) return;
+    (void)hipEventRecord(c->prof_ev[c->prof_n], c->stream);
+    c->prof_stage[c->prof_n] = s;
+    c->prof_n++;
+}
 
 // Supported grid sizes (FFT lengths).
 bool fft_size_supported(int n);

@@ -251,10 +251,15 @@ struct PoissonPasses {
     }
     static void project(mhit_ctx* c, const double* a, const double* b, const double* w, bool div) {
         x_fwd(c, a, b, w, div);
+        prof_mark(c, ST_FFT_X_FWD);
         y_pass<0>(c);
+        prof_mark(c, ST_FFT_Y_FWD);
         z_pass<2>(c);
+        prof_mark(c, ST_FFT_Z);
         y_pass<1>(c);
+        prof_mark(c, ST_FFT_Y_INV);
         x_inv(c);
+        prof_mark(c, ST_FFT_X_INV);
     }
     static void r2c(mhit_ctx* c, const double* in) {
         x_fwd(c, in, nullptr, nullptr, false);

@@ -110,11 +110,15 @@ void k_acdi_update(double* __restrict__ phi, const double* __restrict__ rphi_in,
 int launch_acdi(mhit_ctx* c) {
     CellLaunch L(c->n);
     k_acdi_psi<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->psi, c->tanh_psi, c->g, c->S);
+    prof_mark(c, ST_ACDI_PSI);
     k_acdi_rhs<<<L.grid, L.block, 0, c->stream>>>(c->u, c->v, c->w, c->phi, c->psi, c->rhsphi,
                                                   c->normx, c->normy, c->normz, c->g, c->S);
+    prof_mark(c, ST_ACDI_RHS);
     k_acdi_normalize<<<L.grid, L.block, 0, c->stream>>>(c->normx, c->normy, c->normz, c->g, c->S);
+    prof_mark(c, ST_ACDI_NORM);
     k_acdi_update<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->rhsphi, c->rhsphi_o, c->tanh_psi,
                                                      c->normx, c->normy, c->normz, c->g, c->S);
+    prof_mark(c, ST_ACDI_UPDATE);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
@@ -215,6 +219,7 @@ int launch_ns(mhit_ctx* c) {
     CellLaunch L(c->n);
     if (c->P.phiflag) {
         k_csf<<<L.grid, L.block, 0, c->stream>>>(c->phi, c->fxst, c->fyst, c->fzst, c->g, c->S);
+        prof_mark(c, ST_CSF);
         k_ns<true><<<L.grid, L.block, 0, c->stream>>>(
             c->u, c->v, c->w, c->fxst, c->fyst, c->fzst, c->rhsu_o, c->rhsv_o, c->rhsw_o,
             c->us, c->vs, c->ws, c->mysin, c->mycos, c->g, c->S);
@@ -223,6 +228,7 @@ int launch_ns(mhit_ctx* c) {
             c->u, c->v, c->w, nullptr, nullptr, nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o,
             c->us, c->vs, c->ws, c->mysin, c->mycos, c->g, c->S);
     }
+    prof_mark(c, ST_NS);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
@@ -290,6 +296,7 @@ int launch_correct(mhit_ctx* c) {
     else
         k_correct<double><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const double*)c->pres,
                                                         c->u, c->v, c->w, c->partials, c->g, c->S);
+    prof_mark(c, ST_CORRECT);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
@@ -341,6 +348,7 @@ void k_finalize(const double* __restrict__ partials, int nblocks, Grid g, int cf
 
 int launch_finalize(mhit_ctx* c) {
     k_finalize<<<1, 1024, 0, c->stream>>>(c->partials, c->npart_blocks, c->g, c->P.cfl_abort, c->S);
+    prof_mark(c, ST_FINALIZE);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }

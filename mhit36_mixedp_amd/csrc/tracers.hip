@@ -99,6 +99,7 @@ int launch_tracers(mhit_ctx* c) {
     else
         k_tracers<MHIT_INTERP_REFERENCE><<<blocks, 256, 0, c->stream>>>(
             c->u, c->v, c->w, c->px, c->py, c->pz, c->pu1, c->pv1, c->pw1, c->np, c->g, c->S);
+    prof_mark(c, ST_TRACERS);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
