@@ -4,6 +4,7 @@
 #include "internal.hpp"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -100,7 +101,8 @@ double stage_bytes(const mhit_ctx* c, int s) {
         case ST_ACDI_UPDATE: return 72.0 * N3;               // r phi,rphi,rphi_o,T,n x3; w phi,rphi_o
         case ST_ACDI: return 56.0 * N3;                      // r phi,u,v,w,rphi_o; w phi',rphi_o
         case ST_CSF: return 32.0 * N3;                       // r phi; w f x3
-        case ST_NS: return (c->P.phiflag ? 120.0 : 96.0) * N3; // r u,v,w,rhs_o[,f x3]; w u*,rhs_o
+        case ST_NS:   // r u,v,w,rhs_o[,phi or f x3]; w u*,rhs_o
+            return (c->P.phiflag ? (c->march ? 104.0 : 120.0) : 96.0) * N3;
         case ST_FFT_X_FWD: return 24.0 * N3 + spec;          // r u*,v*,w*; w spectrum
         case ST_FFT_Y_FWD: case ST_FFT_Z: case ST_FFT_Y_INV: return 2.0 * spec;
         case ST_FFT_X_INV: return spec + rs * N3;            // r spectrum; w p
@@ -118,8 +120,8 @@ static int enqueue_step(mhit_ctx* c) {
         prof_mark(c, -1);
     }
     if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
-    if (c->P.phiflag && (rc = launch_acdi(c))) return rc;                    // 521-637
-    if ((rc = launch_ns(c))) return rc;                                      // 658-796
+    if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
+    if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
     if ((rc = poisson_project(c))) return rc;                                // 833-923
     if ((rc = launch_correct(c))) return rc;                                 // 942-962
     if ((rc = launch_finalize(c))) return rc;                                // 964-1007
@@ -192,6 +194,9 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     const int n = c->n = P->nx;
     c->N3 = (size_t)n * n * n;
     c->fp32 = P->poisson_precision == MHIT_POISSON_FP32;
+    // MHIT_STENCIL_PATH=cell selects the one-cell-per-thread kernels that
+    // follow the reference's loop split (a cross-check of the fused passes)
+    if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
     // readinput.f90:39-49
     const double twopi = 8.0 * std::atan(1.0);
     Grid& g = c->g;
@@ -222,7 +227,7 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
         if ((rc = dmalloc(f, N3))) return fail(rc);
     if (P->phiflag) {
         double** pf[] = {&c->phi, &c->rhsphi, &c->rhsphi_o, &c->psi, &c->tanh_psi, &c->normx,
-                         &c->normy, &c->normz, &c->fxst, &c->fyst, &c->fzst};
+                         &c->normy, &c->normz, &c->fxst, &c->fyst, &c->fzst, &c->phi_next};
         for (double** f : pf)
             if ((rc = dmalloc(f, N3))) return fail(rc);
     }
@@ -301,7 +306,7 @@ int mhit_destroy(mhit_ctx* c) {
     if (!c) return MHIT_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->u, c->v, c->w, c->us, c->vs, c->ws, c->rhsu_o, c->rhsv_o, c->rhsw_o,
-                    c->phi, c->rhsphi, c->rhsphi_o, c->psi, c->tanh_psi, c->normx, c->normy,
+                    c->phi, c->phi_next, c->rhsphi, c->rhsphi_o, c->psi, c->tanh_psi, c->normx, c->normy,
                     c->normz, c->fxst, c->fyst, c->fzst, c->pres, c->spec, c->nyq, c->tw, c->kk,
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
                     c->pu1, c->pv1, c->pw1};

@@ -20,6 +20,8 @@ struct mhit_ctx {
     double *rhsu_o = nullptr, *rhsv_o = nullptr, *rhsw_o = nullptr;
     // phase field (module.f90:66-71)
     double *phi = nullptr, *rhsphi = nullptr, *rhsphi_o = nullptr;
+    double *phi_next = nullptr;   // ping-pong target of the fused ACDI pass
+    bool march = true;            // fused z-marching stencil passes (default)
     double *psi = nullptr, *tanh_psi = nullptr;
     double *normx = nullptr, *normy = nullptr, *normz = nullptr;
     double *fxst = nullptr, *fyst = nullptr, *fzst = nullptr;
@@ -85,10 +87,15 @@ int fft_c2r_dev(mhit_ctx* c);                           // spec, nyq -> pres
 // stencil.hip
 int launch_acdi(mhit_ctx* c);
 int launch_ns(mhit_ctx* c);
+
 int launch_correct(mhit_ctx* c);
 int launch_finalize(mhit_ctx* c);
 int launch_diag(mhit_ctx* c, double* host_out /* 8 values */);
 int correct_block_count(int n);
+
+// march.hip
+int launch_acdi_march(mhit_ctx* c);
+int launch_ns_march(mhit_ctx* c);
 
 // tracers.hip
 int launch_tracers(mhit_ctx* c);

@@ -1,0 +1,351 @@
+// march.hip -- fused z-marching stencil passes (gfx950).
+//
+// A workgroup owns a tile of columns (MX x MY, one column per thread) and
+// marches up a chunk of z planes.  Planes enter rings in LDS with their x/y
+// halo; values needed only on the thread's own column stay in registers.
+// Every array is then read from HBM once per pass (halo rows and the few
+// planes before and after a chunk come mostly from L2/MALL), instead of
+// once per neighbour as in a cell-per-thread kernel.
+//
+// NS pass: N1 (main.f90:658-717) + S1 (721-743) + S2 (760-780) or N2
+//   (783-796) in one kernel: reads u, v, w, phi, rhs_o; writes u*, rhs_o.
+// ACDI pass: P1-P5 (main.f90:522-630) in one kernel: reads phi, u, v, w,
+//   rhsphi_o; writes phi', rhsphi_o.  psi = eps log(..) is computed once per
+//   plane on the tile + 2-point halo, the normalised normals and the
+//   sharpening fluxes F_d = 1/4 (1 - tanh^2) n_d once per point of the tile
+//   + 1-point halo.
+// Arithmetic follows the reference expressions term by term; only where
+// each intermediate lives changes.
+#include "internal.hpp"
+
+namespace mhit {
+
+__device__ __forceinline__ int wrapn(int a, int n) {
+    a %= n;
+    return a < 0 ? a + n : a;
+}
+__device__ __forceinline__ int ring3(int k) { return ((k % 3) + 3) % 3; }
+__device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
+
+// =====================================================================
+// NS + CSF + AB2
+// =====================================================================
+constexpr int NS_MX = 64, NS_MY = 8, NS_MZ = 32;
+constexpr int NS_W1 = NS_MX + 2, NS_H1 = NS_MY + 2;   // halo 1 (velocity)
+constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // halo 2 (phi)
+
+template <bool PHI>
+__global__ __launch_bounds__(NS_MX * NS_MY)
+void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
+                const double* __restrict__ w, const double* __restrict__ phi,
+                double* __restrict__ ru_o, double* __restrict__ rv_o, double* __restrict__ rw_o,
+                double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
+                const double* __restrict__ mysin, const double* __restrict__ mycos,
+                Grid g, const DevScalars* __restrict__ S, int zc) {
+    constexpr int MX = NS_MX, MY = NS_MY, W1 = NS_W1, H1 = NS_H1, W2 = NS_W2, H2 = NS_H2;
+    constexpr int NT = MX * MY;
+    __shared__ double su[3][H1 * W1], sv[3][H1 * W1], sw[3][H1 * W1];
+    __shared__ double sp[PHI ? 3 : 1][PHI ? H2 * W2 : 1];
+    __shared__ double sfx[PHI ? MY * (MX + 1) : 1];   // fx at x = -1..MX-1
+    __shared__ double sfy[PHI ? (MY + 1) * MX : 1];   // fy at y = -1..MY-1
+
+    if (S->halted) return;
+    const int n = g.n;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
+    const int i0 = blockIdx.x * MX, j0 = blockIdx.y * MY;
+    const int z0 = blockIdx.z * zc;
+    const int z1 = min(z0 + zc, n);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool live = i < n && j < n;
+    const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
+    const double alpha = S->alpha, beta = S->beta;
+    const double dxi = g.dxi, ddxi = g.ddxi, mu = g.mu, rhoi = g.rhoi, dt = g.dt;
+    const double eps = g.eps, epsi = g.epsi, sig6 = 6.0 * g.sigma;
+
+    auto load_uvw = [&](int k) {
+        const int kk = wrapn(k, n), s = ring3(k);
+        for (int e = tid; e < H1 * W1; e += NT) {
+            const int lx = e % W1, ly = e / W1;
+            const size_t q = idx3(wrapn(i0 - 1 + lx, n), wrapn(j0 - 1 + ly, n), kk, n);
+            su[s][e] = u[q] - m0;
+            sv[s][e] = v[q] - m1;
+            sw[s][e] = w[q] - m2;
+        }
+    };
+    auto load_phi = [&](int k) {
+        const int kk = wrapn(k, n), s = ring3(k);
+        for (int e = tid; e < H2 * W2; e += NT) {
+            const int lx = e % W2, ly = e / W2;
+            sp[s][e] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+        }
+    };
+    // phi at tile coordinates (a, b) of plane k (a, b from -2)
+    auto P = [&](int k, int a, int b) -> double { return sp[ring3(k)][(b + 2) * W2 + (a + 2)]; };
+    // chemical potential at (a, b, k), main.f90:735
+    auto chem = [&](int k, int a, int b) -> double {
+        const double ph = P(k, a, b);
+        return ph * (1.0 - ph) * (1.0 - 2.0 * ph) * epsi -
+               eps * (P(k, a + 1, b) + P(k, a - 1, b) + P(k, a, b + 1) + P(k, a, b - 1) +
+                      P(k + 1, a, b) + P(k - 1, a, b) - 6.0 * ph) * ddxi;
+    };
+
+    double fz_prev = 0.0, fz_cur = 0.0;
+    // warm-up: planes z0-1, z0 of u,v,w; phi planes z0-2..z0 and fz(z0-1)
+    load_uvw(z0 - 1);
+    load_uvw(z0);
+    if (PHI) {
+        load_phi(z0 - 2);
+        load_phi(z0 - 1);
+        load_phi(z0);
+    }
+    __syncthreads();
+    if (PHI) fz_prev = sig6 * chem(z0 - 1, tx, ty) * 0.5 * (P(z0, tx, ty) - P(z0 - 2, tx, ty)) * dxi;
+    __syncthreads();
+
+    for (int k = z0; k < z1; k++) {
+        load_uvw(k + 1);
+        if (PHI) load_phi(k + 1);
+        __syncthreads();
+        if (PHI) {
+            // S1 at plane k: fx on x = -1..MX-1, fy on y = -1..MY-1, fz on the column
+            const double c0 = chem(k, tx, ty);
+            sfx[ty * (MX + 1) + tx + 1] = sig6 * c0 * 0.5 * (P(k, tx + 1, ty) - P(k, tx - 1, ty)) * dxi;
+            sfy[(ty + 1) * MX + tx] = sig6 * c0 * 0.5 * (P(k, tx, ty + 1) - P(k, tx, ty - 1)) * dxi;
+            fz_cur = sig6 * c0 * 0.5 * (P(k + 1, tx, ty) - P(k - 1, tx, ty)) * dxi;
+            if (tid < MY) {            // x = -1 column
+                const int b = tid;
+                sfx[b * (MX + 1)] = sig6 * chem(k, -1, b) * 0.5 * (P(k, 0, b) - P(k, -2, b)) * dxi;
+            } else if (tid < MY + MX) { // y = -1 row
+                const int a = tid - MY;
+                sfy[a] = sig6 * chem(k, a, -1) * 0.5 * (P(k, a, 0) - P(k, a, -2)) * dxi;
+            }
+            __syncthreads();
+        }
+        const int sm = ring3(k - 1), sc = ring3(k), sq = ring3(k + 1);
+        const int c = (ty + 1) * W1 + (tx + 1);
+        const double u0 = su[sc][c], v0 = sv[sc][c], w0 = sw[sc][c];
+        const double uxp = su[sc][c + 1], uxm = su[sc][c - 1], uyp = su[sc][c + W1], uym = su[sc][c - W1];
+        const double uzp = su[sq][c], uzm = su[sm][c];
+        const double vxp = sv[sc][c + 1], vxm = sv[sc][c - 1], vyp = sv[sc][c + W1], vym = sv[sc][c - W1];
+        const double vzp = sv[sq][c], vzm = sv[sm][c];
+        const double wxp = sw[sc][c + 1], wxm = sw[sc][c - 1], wyp = sw[sc][c + W1], wym = sw[sc][c - W1];
+        const double wzp = sw[sq][c], wzm = sw[sm][c];
+        double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+        double h12 = (uyp + u0) * (vyp + sv[sc][c + W1 - 1]) - (u0 + uym) * (v0 + vxm);
+        double h13 = (uzp + u0) * (wzp + sw[sq][c - 1]) - (u0 + uzm) * (w0 + wxm);
+        double h21 = (uxp + su[sc][c + 1 - W1]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+        double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+        double h23 = (wzp + sw[sq][c - W1]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
+        double h31 = (wxp + w0) * (uxp + su[sm][c + 1]) - (w0 + wxm) * (u0 + uzm);
+        double h32 = (vyp + sv[sm][c + W1]) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
+        double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+        h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
+        h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
+        h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
+        double ru = -(h11 + h12 + h13);
+        double rv = -(h21 + h22 + h23);
+        double rw = -(h31 + h32 + h33);
+        h11 = mu * (uxp - 2.0 * u0 + uxm) * ddxi;
+        h12 = mu * (uyp - 2.0 * u0 + uym) * ddxi;
+        h13 = mu * (uzp - 2.0 * u0 + uzm) * ddxi;
+        h21 = mu * (vxp - 2.0 * v0 + vxm) * ddxi;
+        h22 = mu * (vyp - 2.0 * v0 + vym) * ddxi;
+        h23 = mu * (vzp - 2.0 * v0 + vzm) * ddxi;
+        h31 = mu * (wxp - 2.0 * w0 + wxm) * ddxi;
+        h32 = mu * (wyp - 2.0 * w0 + wym) * ddxi;
+        h33 = mu * (wzp - 2.0 * w0 + wzm) * ddxi;
+        ru = ru + (h11 + h12 + h13) * rhoi;
+        rv = rv + (h21 + h22 + h23) * rhoi;
+        rw = rw + (h31 + h32 + h33) * rhoi;
+        if (live) {
+            ru = ru + g.f3 * mysin[k + 1] + g.f2 * mycos[j + 1];
+            rv = rv + g.f1 * mysin[i] + g.f3 * mycos[k + 1];
+            rw = rw + g.f2 * mysin[j + 1] + g.f1 * mycos[i];
+            if (PHI) {
+                ru = ru + 0.5 * (sfx[ty * (MX + 1) + tx] + sfx[ty * (MX + 1) + tx + 1]) * rhoi;
+                rv = rv + 0.5 * (sfy[ty * MX + tx] + sfy[(ty + 1) * MX + tx]) * rhoi;
+                rw = rw + 0.5 * (fz_prev + fz_cur) * rhoi;
+            }
+            const size_t q = idx3(i, j, k, n);
+            us[q] = u0 + dt * (alpha * ru - beta * ru_o[q]);
+            vs[q] = v0 + dt * (alpha * rv - beta * rv_o[q]);
+            ws[q] = w0 + dt * (alpha * rw - beta * rw_o[q]);
+            ru_o[q] = ru;
+            rv_o[q] = rv;
+            rw_o[q] = rw;
+        }
+        fz_prev = fz_cur;
+        __syncthreads();
+    }
+}
+
+// =====================================================================
+// ACDI
+// =====================================================================
+constexpr int AC_MX = 64, AC_MY = 16, AC_MZ = 32;
+constexpr int AC_W1 = AC_MX + 2, AC_H1 = AC_MY + 2;   // phi ring: halo 1
+constexpr int AC_W2 = AC_MX + 4, AC_H2 = AC_MY + 4;   // psi ring: halo 2
+
+__global__ __launch_bounds__(AC_MX * AC_MY)
+void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
+                  const double* __restrict__ u, const double* __restrict__ v,
+                  const double* __restrict__ w, double* __restrict__ rphi_o,
+                  Grid g, const DevScalars* __restrict__ S, int zc) {
+    constexpr int MX = AC_MX, MY = AC_MY, W1 = AC_W1, H1 = AC_H1, W2 = AC_W2, H2 = AC_H2;
+    constexpr int NT = MX * MY;
+    __shared__ double spsi[3][H2 * W2];
+    __shared__ double sphi[4][H1 * W1];
+    __shared__ double sFx[2][MY * (MX + 2)];   // x = -1..MX
+    __shared__ double sFy[2][(MY + 2) * MX];   // y = -1..MY
+
+    if (S->halted) return;
+    const int n = g.n;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
+    const int i0 = blockIdx.x * MX, j0 = blockIdx.y * MY;
+    const int z0 = blockIdx.z * zc;
+    const int z1 = min(z0 + zc, n);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool live = i < n && j < n;
+    const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
+    const double alpha = S->alpha, beta = S->beta;
+    const double gamma = 1.0 * S->gumax;               // main.f90:536
+    const double dxi = g.dxi, ddxi = g.ddxi, eps = g.eps, epsi = g.epsi, enm = g.enm, dt = g.dt;
+
+    // phi plane k -> psi ring (halo 2) and phi ring (halo 1), main.f90:522-534
+    auto load_plane = [&](int k) {
+        const int kk = wrapn(k, n), sps = ring3(k), sph = ring4(k);
+        for (int e = tid; e < H2 * W2; e += NT) {
+            const int lx = e % W2, ly = e / W2;
+            const double ph = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+            const double val = fmin(ph, 1.0);
+            spsi[sps][e] = eps * log((val + enm) / (1.0 - val + enm));
+            if (lx >= 1 && lx <= MX + 2 && ly >= 1 && ly <= MY + 2)
+                sphi[sph][(ly - 1) * W1 + (lx - 1)] = ph;
+        }
+    };
+    auto PS = [&](int k, int a, int b) -> double { return spsi[ring3(k)][(b + 2) * W2 + (a + 2)]; };
+    auto PH = [&](int k, int a, int b) -> double { return sphi[ring4(k)][(b + 1) * W1 + (a + 1)]; };
+    // normalised normal and 1/4 (1 - tanh^2) at (a, b, k): main.f90:530, 559-561, 586-589
+    auto flux = [&](int k, int a, int b, double& fx, double& fy, double& fz) {
+        const double nx = (PS(k, a + 1, b) - PS(k, a - 1, b));
+        const double ny = (PS(k, a, b + 1) - PS(k, a, b - 1));
+        const double nz = (PS(k + 1, a, b) - PS(k - 1, a, b));
+        const double normag = 1.0 / (sqrt(nx * nx + ny * ny + nz * nz) + enm);
+        const double T = tanh(0.5 * PS(k, a, b) * epsi);
+        const double G = 0.25 * (1.0 - T * T);
+        fx = G * (nx * normag);
+        fy = G * (ny * normag);
+        fz = G * (nz * normag);
+    };
+    // sharpening fluxes of plane k into buffer bf (tile + 1-point halo in x / y);
+    // returns Fz at the thread's column
+    auto fluxes = [&](int k, int bf) -> double {
+        double fx, fy, fz;
+        flux(k, tx, ty, fx, fy, fz);
+        sFx[bf][ty * (MX + 2) + tx + 1] = fx;
+        sFy[bf][(ty + 1) * MX + tx] = fy;
+        if (tid < 2 * MY) {                 // x = -1 and x = MX columns
+            const int b = tid >> 1, a = (tid & 1) ? MX : -1;
+            double gx, gy, gz;
+            flux(k, a, b, gx, gy, gz);
+            sFx[bf][b * (MX + 2) + a + 1] = gx;
+        } else if (tid < 2 * MY + 2 * MX) { // y = -1 and y = MY rows
+            const int e = tid - 2 * MY;
+            const int a = e >> 1, b = (e & 1) ? MY : -1;
+            double gx, gy, gz;
+            flux(k, a, b, gx, gy, gz);
+            sFy[bf][(b + 1) * MX + a] = gy;
+        }
+        return fz;
+    };
+
+    // warm-up: psi planes z0-2..z0 -> Fz(z0-1); plane z0+1 -> fluxes of z0
+    load_plane(z0 - 2);
+    load_plane(z0 - 1);
+    load_plane(z0);
+    __syncthreads();
+    double fz_m1 = 0.0, fz_0 = 0.0, fz_p1 = 0.0;
+    {
+        double fx, fy;
+        flux(z0 - 1, tx, ty, fx, fy, fz_m1);
+    }
+    __syncthreads();
+    load_plane(z0 + 1);
+    __syncthreads();
+    int bf = 0;
+    fz_0 = fluxes(z0, bf);
+    __syncthreads();
+
+    const size_t zstride = (size_t)n * n;
+    double wk = 0.0;
+    if (live) wk = w[idx3(i, j, z0, n)] - m2;
+    for (int k = z0; k < z1; k++) {
+        load_plane(k + 2);
+        __syncthreads();
+        fz_p1 = fluxes(k + 1, bf ^ 1);
+        if (live) {
+            const size_t q = idx3(i, j, k, n);
+            const double pc = PH(k, tx, ty);
+            const double pxp = PH(k, tx + 1, ty), pxm = PH(k, tx - 1, ty);
+            const double pyp = PH(k, tx, ty + 1), pym = PH(k, tx, ty - 1);
+            const double pzp = PH(k + 1, tx, ty), pzm = PH(k - 1, tx, ty);
+            const double uc = u[q] - m0, up = u[idx3(wrap_p(i, n), j, k, n)] - m0;
+            const double vc = v[q] - m1, vp = v[idx3(i, wrap_p(j, n), k, n)] - m1;
+            const double wp = w[idx3(i, j, wrap_p(k, n), n)] - m2;
+            (void)zstride;
+            // P2 conv + diff, main.f90:551-557
+            double r = -(up * 0.5 * (pxp + pc) - uc * 0.5 * (pc + pxm)) * dxi
+                       - (vp * 0.5 * (pyp + pc) - vc * 0.5 * (pc + pym)) * dxi
+                       - (wp * 0.5 * (pzp + pc) - wk * 0.5 * (pc + pzm)) * dxi
+                       + gamma * (eps * (pxp - 2.0 * pc + pxm) * ddxi +
+                                  eps * (pyp - 2.0 * pc + pym) * ddxi +
+                                  eps * (pzp - 2.0 * pc + pzm) * ddxi);
+            // P4 sharpening, main.f90:609-614
+            const double* Fx = sFx[bf];
+            const double* Fy = sFy[bf];
+            r = r - gamma * ((Fx[ty * (MX + 2) + tx + 2] - Fx[ty * (MX + 2) + tx]) * 0.5 * dxi +
+                             (Fy[(ty + 2) * MX + tx] - Fy[ty * MX + tx]) * 0.5 * dxi +
+                             (fz_p1 - fz_m1) * 0.5 * dxi);
+            // P5 AB2, main.f90:625-626
+            phi_out[q] = pc + dt * (alpha * r - beta * rphi_o[q]);
+            rphi_o[q] = r;
+            wk = wp;
+        }
+        fz_m1 = fz_0;
+        fz_0 = fz_p1;
+        bf ^= 1;
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------- launchers
+int launch_ns_march(mhit_ctx* c) {
+    const int n = c->n;
+    const int zc = n < NS_MZ ? n : NS_MZ;
+    dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, zc)), block(NS_MX, NS_MY, 1);
+    if (c->P.phiflag)
+        k_ns_march<true><<<grid, block, 0, c->stream>>>(
+            c->u, c->v, c->w, c->phi, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
+            c->mysin, c->mycos, c->g, c->S, zc);
+    else
+        k_ns_march<false><<<grid, block, 0, c->stream>>>(
+            c->u, c->v, c->w, nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
+            c->mysin, c->mycos, c->g, c->S, zc);
+    prof_mark(c, ST_NS);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+int launch_acdi_march(mhit_ctx* c) {
+    const int n = c->n;
+    const int zc = n < AC_MZ ? n : AC_MZ;
+    dim3 grid(div_up(n, AC_MX), div_up(n, AC_MY), div_up(n, zc)), block(AC_MX, AC_MY, 1);
+    k_acdi_march<<<grid, block, 0, c->stream>>>(c->phi_next, c->phi, c->u, c->v, c->w, c->rhsphi_o,
+                                                c->g, c->S, zc);
+    prof_mark(c, ST_ACDI);
+    MHIT_HIP_TRY(hipGetLastError());
+    std::swap(c->phi, c->phi_next);
+    return MHIT_OK;
+}
+
+}  // namespace mhit

@@ -90,9 +90,13 @@ def test_ns_many_steps_fp64(oracle_mod):
         assert ds[k] == pytest.approx(do[k], rel=1e-10), k
 
 
+@pytest.mark.parametrize("path", ["march", "cell"])
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
-def test_multiphase_tracers_steps(prec, oracle_mod):
-    print(f"[multiphase 10 steps] prec={prec}", flush=True)
+def test_multiphase_tracers_steps(prec, path, oracle_mod, monkeypatch):
+    """Both GPU stencil paths: the fused z-marching passes (default) and the
+    cell-per-thread kernels split as the reference's loops."""
+    print(f"[multiphase 10 steps] prec={prec} path={path}", flush=True)
+    monkeypatch.setenv("MHIT_STENCIL_PATH", path)
     s, o = make_pair(oracle_mod, 32, prec, phiflag=1, partflag=1, ndrops=3, radius=0.6,
                      ntr=2000, sigma=0.5)
     s.step(1)
@@ -137,3 +141,17 @@ def test_cfl_abort():
         assert d["cou"] > 7
         with pytest.raises(CFLStop):
             s.step(1)
+
+
+@pytest.mark.parametrize("n", [96, 24])
+def test_multiphase_partial_tiles_and_chunks(n, oracle_mod):
+    """Sizes that are not multiples of the 64 x 8 / 64 x 16 tiles and give
+    several 32-plane z chunks (96) or fewer planes than a chunk (24)."""
+    print(f"[multiphase n={n}]", flush=True)
+    s, o = make_pair(oracle_mod, n, 0, phiflag=1, partflag=1, ndrops=4, radius=0.5, ntr=500,
+                     sigma=0.4, kmax=5.0)
+    s.step(3)
+    assert o.step(3) == 0
+    e = compare(s, o, phi=True, tr=True)
+    print(e, flush=True)
+    assert max(e.values()) < 1e-11, e
