@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--we", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-Poisson form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()

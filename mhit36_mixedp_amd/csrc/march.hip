@@ -62,21 +62,44 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     const double dxi = g.dxi, ddxi = g.ddxi, mu = g.mu, rhoi = g.rhoi, dt = g.dt;
     const double eps = g.eps, epsi = g.epsi, sig6 = 6.0 * g.sigma;
 
-    auto load_uvw = [&](int k) {
-        const int kk = wrapn(k, n), s = ring3(k);
-        for (int e = tid; e < H1 * W1; e += NT) {
-            const int lx = e % W1, ly = e / W1;
-            const size_t q = idx3(wrapn(i0 - 1 + lx, n), wrapn(j0 - 1 + ly, n), kk, n);
-            su[s][e] = u[q] - m0;
-            sv[s][e] = v[q] - m1;
-            sw[s][e] = w[q] - m2;
+    // one plane in flight: global -> registers (fetch), registers -> ring (commit)
+    constexpr int NV = (H1 * W1 + NT - 1) / NT, NPH = (H2 * W2 + NT - 1) / NT;
+    double pu[NV], pv[NV], pw[NV], pp[NPH];
+    auto fetch = [&](int k) {
+        const int kk = wrapn(k, n);
+#pragma unroll
+        for (int t = 0; t < NV; t++) {
+            const int e = tid + t * NT;
+            if (e < H1 * W1) {
+                const int lx = e % W1, ly = e / W1;
+                const size_t q = idx3(wrapn(i0 - 1 + lx, n), wrapn(j0 - 1 + ly, n), kk, n);
+                pu[t] = u[q]; pv[t] = v[q]; pw[t] = w[q];
+            }
+        }
+        if (PHI) {
+#pragma unroll
+            for (int t = 0; t < NPH; t++) {
+                const int e = tid + t * NT;
+                if (e < H2 * W2) {
+                    const int lx = e % W2, ly = e / W2;
+                    pp[t] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+                }
+            }
         }
     };
-    auto load_phi = [&](int k) {
-        const int kk = wrapn(k, n), s = ring3(k);
-        for (int e = tid; e < H2 * W2; e += NT) {
-            const int lx = e % W2, ly = e / W2;
-            sp[s][e] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+    auto commit = [&](int k) {
+        const int s = ring3(k);
+#pragma unroll
+        for (int t = 0; t < NV; t++) {
+            const int e = tid + t * NT;
+            if (e < H1 * W1) { su[s][e] = pu[t] - m0; sv[s][e] = pv[t] - m1; sw[s][e] = pw[t] - m2; }
+        }
+        if (PHI) {
+#pragma unroll
+            for (int t = 0; t < NPH; t++) {
+                const int e = tid + t * NT;
+                if (e < H2 * W2) sp[s][e] = pp[t];
+            }
         }
     };
     // phi at tile coordinates (a, b) of plane k (a, b from -2)
@@ -91,20 +114,25 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
 
     double fz_prev = 0.0, fz_cur = 0.0;
     // warm-up: planes z0-1, z0 of u,v,w; phi planes z0-2..z0 and fz(z0-1)
-    load_uvw(z0 - 1);
-    load_uvw(z0);
     if (PHI) {
-        load_phi(z0 - 2);
-        load_phi(z0 - 1);
-        load_phi(z0);
+        fetch(z0 - 2);
+        commit(z0 - 2);   // also fills the velocity slot of z0-2 = z0+1, rewritten below
     }
+    fetch(z0 - 1);
+    commit(z0 - 1);
+    fetch(z0);
+    commit(z0);
     __syncthreads();
     if (PHI) fz_prev = sig6 * chem(z0 - 1, tx, ty) * 0.5 * (P(z0, tx, ty) - P(z0 - 2, tx, ty)) * dxi;
     __syncthreads();
+    fetch(z0 + 1);
 
     for (int k = z0; k < z1; k++) {
-        load_uvw(k + 1);
-        if (PHI) load_phi(k + 1);
+        // AB2 history of the own column: issued before the barriers
+        const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
+        double rou = 0.0, rov = 0.0, row = 0.0;
+        if (live) { rou = ru_o[q]; rov = rv_o[q]; row = rw_o[q]; }
+        commit(k + 1);
         __syncthreads();
         if (PHI) {
             // S1 at plane k: fx on x = -1..MX-1, fy on y = -1..MY-1, fz on the column
@@ -121,6 +149,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
             }
             __syncthreads();
         }
+        if (k + 2 <= z1) fetch(k + 2);   // next plane in flight during the compute
         const int sm = ring3(k - 1), sc = ring3(k), sq = ring3(k + 1);
         const int c = (ty + 1) * W1 + (tx + 1);
         const double u0 = su[sc][c], v0 = sv[sc][c], w0 = sw[sc][c];
@@ -166,10 +195,9 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
                 rv = rv + 0.5 * (sfy[ty * MX + tx] + sfy[(ty + 1) * MX + tx]) * rhoi;
                 rw = rw + 0.5 * (fz_prev + fz_cur) * rhoi;
             }
-            const size_t q = idx3(i, j, k, n);
-            us[q] = u0 + dt * (alpha * ru - beta * ru_o[q]);
-            vs[q] = v0 + dt * (alpha * rv - beta * rv_o[q]);
-            ws[q] = w0 + dt * (alpha * rw - beta * rw_o[q]);
+            us[q] = u0 + dt * (alpha * ru - beta * rou);
+            vs[q] = v0 + dt * (alpha * rv - beta * rov);
+            ws[q] = w0 + dt * (alpha * rw - beta * row);
             ru_o[q] = ru;
             rv_o[q] = rv;
             rw_o[q] = rw;
@@ -182,7 +210,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
 // =====================================================================
 // ACDI
 // =====================================================================
-constexpr int AC_MX = 64, AC_MY = 16, AC_MZ = 32;
+constexpr int AC_MX = 64, AC_MY = 16, AC_MZ = 64;
 constexpr int AC_W1 = AC_MX + 2, AC_H1 = AC_MY + 2;   // phi ring: halo 1
 constexpr int AC_W2 = AC_MX + 4, AC_H2 = AC_MY + 4;   // psi ring: halo 2
 
@@ -209,14 +237,31 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
     const double alpha = S->alpha, beta = S->beta;
     const double gamma = 1.0 * S->gumax;               // main.f90:536
-    const double dxi = g.dxi, ddxi = g.ddxi, eps = g.eps, epsi = g.epsi, enm = g.enm, dt = g.dt;
+    const double dxi = g.dxi, ddxi = g.ddxi, eps = g.eps, enm = g.enm, dt = g.dt;
 
-    // phi plane k -> psi ring (halo 2) and phi ring (halo 1), main.f90:522-534
-    auto load_plane = [&](int k) {
-        const int kk = wrapn(k, n), sps = ring3(k), sph = ring4(k);
-        for (int e = tid; e < H2 * W2; e += NT) {
+    // phi of one plane in flight: global -> registers
+    constexpr int NPL = (H2 * W2 + NT - 1) / NT;
+    double pf[NPL];
+    auto fetch = [&](int k) {
+        const int kk = wrapn(k, n);
+#pragma unroll
+        for (int t = 0; t < NPL; t++) {
+            const int e = tid + t * NT;
+            if (e < H2 * W2) {
+                const int lx = e % W2, ly = e / W2;
+                pf[t] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+            }
+        }
+    };
+    // registers -> psi ring (halo 2) and phi ring (halo 1), main.f90:522-534
+    auto commit = [&](int k) {
+        const int sps = ring3(k), sph = ring4(k);
+#pragma unroll
+        for (int t = 0; t < NPL; t++) {
+            const int e = tid + t * NT;
+            if (e >= H2 * W2) break;
             const int lx = e % W2, ly = e / W2;
-            const double ph = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
+            const double ph = pf[t];
             const double val = fmin(ph, 1.0);
             spsi[sps][e] = eps * log((val + enm) / (1.0 - val + enm));
             if (lx >= 1 && lx <= MX + 2 && ly >= 1 && ly <= MY + 2)
@@ -225,14 +270,20 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     };
     auto PS = [&](int k, int a, int b) -> double { return spsi[ring3(k)][(b + 2) * W2 + (a + 2)]; };
     auto PH = [&](int k, int a, int b) -> double { return sphi[ring4(k)][(b + 1) * W1 + (a + 1)]; };
-    // normalised normal and 1/4 (1 - tanh^2) at (a, b, k): main.f90:530, 559-561, 586-589
+    // normalised normal and 1/4 (1 - tanh^2(psi/(2 eps))) at (a, b, k):
+    // main.f90:530, 559-561, 586-589.  With r = (v+e)/(1-v+e) and
+    // psi = eps ln r, tanh(ln(r)/2) = (r-1)/(r+1) = (2v-1)/(1+2e), hence
+    // 1/4 (1 - tanh^2) = (v+e)(1-v+e)/(1+2e)^2: the same quantity without a
+    // second transcendental, and without the cancellation of 1 - T*T where
+    // phi is near 0 or 1.
+    const double g_scale = 1.0 / ((1.0 + 2.0 * enm) * (1.0 + 2.0 * enm));
     auto flux = [&](int k, int a, int b, double& fx, double& fy, double& fz) {
         const double nx = (PS(k, a + 1, b) - PS(k, a - 1, b));
         const double ny = (PS(k, a, b + 1) - PS(k, a, b - 1));
         const double nz = (PS(k + 1, a, b) - PS(k - 1, a, b));
         const double normag = 1.0 / (sqrt(nx * nx + ny * ny + nz * nz) + enm);
-        const double T = tanh(0.5 * PS(k, a, b) * epsi);
-        const double G = 0.25 * (1.0 - T * T);
+        const double val = fmin(PH(k, a, b), 1.0);
+        const double G = (val + enm) * (1.0 - val + enm) * g_scale;
         fx = G * (nx * normag);
         fy = G * (ny * normag);
         fz = G * (nz * normag);
@@ -260,9 +311,10 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     };
 
     // warm-up: psi planes z0-2..z0 -> Fz(z0-1); plane z0+1 -> fluxes of z0
-    load_plane(z0 - 2);
-    load_plane(z0 - 1);
-    load_plane(z0);
+    for (int k = z0 - 2; k <= z0; k++) {
+        fetch(k);
+        commit(k);
+    }
     __syncthreads();
     double fz_m1 = 0.0, fz_0 = 0.0, fz_p1 = 0.0;
     {
@@ -270,29 +322,37 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
         flux(z0 - 1, tx, ty, fx, fy, fz_m1);
     }
     __syncthreads();
-    load_plane(z0 + 1);
+    fetch(z0 + 1);
+    commit(z0 + 1);
     __syncthreads();
     int bf = 0;
     fz_0 = fluxes(z0, bf);
+    fetch(z0 + 2);
     __syncthreads();
 
-    const size_t zstride = (size_t)n * n;
     double wk = 0.0;
     if (live) wk = w[idx3(i, j, z0, n)] - m2;
     for (int k = z0; k < z1; k++) {
-        load_plane(k + 2);
+        // velocity and AB2 history of the own column: issued before the barriers
+        const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
+        double uc = 0.0, up = 0.0, vc = 0.0, vp = 0.0, wp = 0.0, ro = 0.0;
+        if (live) {
+            uc = u[q] - m0;
+            up = u[idx3(wrap_p(i, n), j, k, n)] - m0;
+            vc = v[q] - m1;
+            vp = v[idx3(i, wrap_p(j, n), k, n)] - m1;
+            wp = w[idx3(i, j, wrap_p(k, n), n)] - m2;
+            ro = rphi_o[q];
+        }
+        commit(k + 2);
         __syncthreads();
+        if (k + 3 <= z1 + 1) fetch(k + 3);   // next plane in flight during the compute
         fz_p1 = fluxes(k + 1, bf ^ 1);
         if (live) {
-            const size_t q = idx3(i, j, k, n);
             const double pc = PH(k, tx, ty);
             const double pxp = PH(k, tx + 1, ty), pxm = PH(k, tx - 1, ty);
             const double pyp = PH(k, tx, ty + 1), pym = PH(k, tx, ty - 1);
             const double pzp = PH(k + 1, tx, ty), pzm = PH(k - 1, tx, ty);
-            const double uc = u[q] - m0, up = u[idx3(wrap_p(i, n), j, k, n)] - m0;
-            const double vc = v[q] - m1, vp = v[idx3(i, wrap_p(j, n), k, n)] - m1;
-            const double wp = w[idx3(i, j, wrap_p(k, n), n)] - m2;
-            (void)zstride;
             // P2 conv + diff, main.f90:551-557
             double r = -(up * 0.5 * (pxp + pc) - uc * 0.5 * (pc + pxm)) * dxi
                        - (vp * 0.5 * (pyp + pc) - vc * 0.5 * (pc + pym)) * dxi
@@ -307,7 +367,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
                              (Fy[(ty + 2) * MX + tx] - Fy[ty * MX + tx]) * 0.5 * dxi +
                              (fz_p1 - fz_m1) * 0.5 * dxi);
             // P5 AB2, main.f90:625-626
-            phi_out[q] = pc + dt * (alpha * r - beta * rphi_o[q]);
+            phi_out[q] = pc + dt * (alpha * r - beta * ro);
             rphi_o[q] = r;
             wk = wp;
         }
