@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-Poisson form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-copy-probe", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -115,6 +116,9 @@ def main():
         fields = {f: s.download_field(f) for f in (FIELD_U, FIELD_V, FIELD_W, FIELD_PHI)}
         t = s.download_tracers()
         tracers = (t["x"], t["y"], t["z"])
+    copy_gbps = s.copy_probe(1 << 30, 10) if not args.no_copy_probe else None
+    if copy_gbps:
+        log(f"[bench] plain device copy: {copy_gbps:.0f} GB/s (read + write)")
     ms32 = time_steps(s, args.warmup, args.steps)
     log(f"[bench] FP32 Poisson: {ms32:.3f} ms/step")
     s.profile_enable(True)
@@ -198,6 +202,7 @@ def main():
             "traffic": None,
             "avg_ms": round(dom_avg_ms, 4),
             "bytes_per_launch": dom["bytes_per_call"],
+            "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
         "cpu_baseline": cpu,

@@ -203,6 +203,12 @@ typedef struct mhit_profile {
 int mhit_profile_enable(mhit_ctx* ctx, int on);   /* also resets counters */
 int mhit_profile_read(mhit_ctx* ctx, mhit_profile* out);
 
+/* Plain device-to-device copy of `bytes` bytes (a read and a write per
+ * byte), repeated iters times on the context's stream; *gbps = achieved
+ * (read + write) GB/s.  The measured copy ceiling of the GPU, and the
+ * calibration kernel for HBM byte counters.  No reference counterpart.    */
+int mhit_copy_probe(mhit_ctx* ctx, int64_t bytes, int iters, double* gbps);
+
 /* Library version string.                                                 */
 const char* mhit_version(void);
 

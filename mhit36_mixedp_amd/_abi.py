@@ -163,6 +163,7 @@ EXPORTS = [
     ("mhit_fft_c2r", ctypes.c_int, [_vp, _dp, _dp]),
     ("mhit_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("mhit_profile_read", ctypes.c_int, [_vp, _P(Profile)]),
+    ("mhit_copy_probe", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, _dp]),
     ("mhit_version", ctypes.c_char_p, []),
 ]
 

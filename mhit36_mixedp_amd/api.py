@@ -137,6 +137,13 @@ class Solver:
     def profile_enable(self, on: bool = True):
         _check(self.lib.mhit_profile_enable(self._h, int(on)), "mhit_profile_enable")
 
+    def copy_probe(self, nbytes: int = 1 << 30, iters: int = 10) -> float:
+        """Plain device copy rate in GB/s (bytes read + written)."""
+        g = ctypes.c_double()
+        _check(self.lib.mhit_copy_probe(self._h, int(nbytes), int(iters), ctypes.byref(g)),
+               "mhit_copy_probe")
+        return g.value
+
     def profile(self) -> list:
         """Per-stage event timing accumulated since profile_enable()."""
         p = _abi.Profile()

@@ -69,6 +69,28 @@ static int push_scalars(mhit_ctx* c) {
     return MHIT_OK;
 }
 
+// "key=value,key=value" with keys ns_mz, ac_mz, xcd, sort; unknown keys and
+// non-positive chunk lengths are ignored.
+static void parse_tuning(const char* s, Tuning& t) {
+    std::string str(s);
+    size_t pos = 0;
+    while (pos < str.size()) {
+        size_t end = str.find(',', pos);
+        if (end == std::string::npos) end = str.size();
+        const std::string kv = str.substr(pos, end - pos);
+        const size_t eq = kv.find('=');
+        if (eq != std::string::npos) {
+            const std::string key = kv.substr(0, eq);
+            const int val = std::atoi(kv.c_str() + eq + 1);
+            if (key == "ns_mz" && val > 0) t.ns_mz = val;
+            else if (key == "ac_mz" && val > 0) t.ac_mz = val;
+            else if (key == "xcd") t.xcd = val != 0;
+            else if (key == "sort" && val >= 0) t.sort_every = val;
+        }
+        pos = end + 1;
+    }
+}
+
 static double* state_field(mhit_ctx* c, int fieldn) {
     switch (fieldn) {
         case MHIT_FIELD_U: return c->u;
@@ -81,7 +103,8 @@ static double* state_field(mhit_ctx* c, int fieldn) {
 
 const char* stage_name(int s) {
     static const char* names[ST_COUNT] = {
-        "tracers", "acdi_psi", "acdi_rhs", "acdi_normals", "acdi_update", "acdi", "csf", "ns_ab2",
+        "tracer_sort", "tracers", "acdi_psi", "acdi_rhs", "acdi_normals", "acdi_update", "acdi",
+        "csf", "ns_ab2",
         "fft_x_fwd_div", "fft_y_fwd", "fft_z_solve", "fft_y_inv", "fft_x_inv", "correct",
         "finalize"};
     return (s >= 0 && s < ST_COUNT) ? names[s] : "?";
@@ -94,6 +117,7 @@ double stage_bytes(const mhit_ctx* c, int s) {
     const double cs = c->fp32 ? 8.0 : 16.0, rs = c->fp32 ? 4.0 : 8.0;
     const double spec = (n / 2) * n * n * cs + n * n * cs;   // half spectrum + Nyquist plane
     switch (s) {
+        case ST_TRACER_SORT: return 120.0 * (double)c->np;   // keys, sort, 7 columns r/w
         case ST_TRACERS: return 144.0 * (double)c->np;       // pos r/w, u_old r/w, 6 gathers
         case ST_ACDI_PSI: return 24.0 * N3;                  // r phi; w psi, tanh
         case ST_ACDI_RHS: return 72.0 * N3;                  // r u,v,w,phi,psi; w rphi, n x3
@@ -119,6 +143,12 @@ static int enqueue_step(mhit_ctx* c) {
         c->prof_n = 0;
         prof_mark(c, -1);
     }
+    if (c->P.partflag && c->np > 1 && c->tune.sort_every > 0 &&
+        c->steps_since_sort >= c->tune.sort_every) {
+        if ((rc = sort_tracers(c))) return rc;
+        c->steps_since_sort = 0;
+    }
+    c->steps_since_sort++;
     if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
     if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
     if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
@@ -139,6 +169,7 @@ static int enqueue_step(mhit_ctx* c) {
 }
 
 static int ensure_tracers(mhit_ctx* c, int64_t n) {
+    c->steps_since_sort = c->tune.sort_every;   // re-order on the next step
     if (n <= c->np_cap) { c->np = n; return MHIT_OK; }
     double** cols[] = {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1};
     for (double** col : cols) { dfree(*col); *col = nullptr; }
@@ -197,6 +228,7 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     // MHIT_STENCIL_PATH=cell selects the one-cell-per-thread kernels that
     // follow the reference's loop split (a cross-check of the fused passes)
     if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
+    if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
     // readinput.f90:39-49
     const double twopi = 8.0 * std::atan(1.0);
     Grid& g = c->g;
@@ -244,7 +276,8 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     c->xext = c->mysin + 2 * (n + 1);
     c->npart_blocks = correct_block_count(n);
     if ((rc = dmalloc(&c->partials, (size_t)c->npart_blocks * 6))) return fail(rc);
-    if ((rc = dmalloc(&c->dpartials, (size_t)c->npart_blocks * 8))) return fail(rc);
+    c->ndiag_blocks = diag_block_count(n);
+    if ((rc = dmalloc(&c->dpartials, (size_t)c->ndiag_blocks * 8))) return fail(rc);
     if ((rc = dmalloc(&c->S, 1))) return fail(rc);
 
     // host tables: grid by cumulative sum and x_ext(n+1) = lx (main.f90:210-217),
@@ -311,6 +344,7 @@ int mhit_destroy(mhit_ctx* c) {
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
                     c->pu1, c->pv1, c->pw1};
     for (void* p : ptrs) dfree(p);
+    free_sort_buffers(c);
     for (hipEvent_t& e : c->prof_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -443,6 +477,37 @@ int mhit_profile_enable(mhit_ctx* c, int on) {
     c->prof = on != 0;
     for (int s = 0; s < MHIT_PROF_MAX; s++) { c->prof_ms[s] = 0.0; c->prof_calls[s] = 0; }
     return MHIT_OK;
+}
+
+int mhit_copy_probe(mhit_ctx* c, int64_t bytes, int iters, double* gbps) {
+    if (!c || bytes < 16 || iters < 1 || !gbps) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    bytes -= bytes % 16;
+    void *src = nullptr, *dst = nullptr;
+    int rc;
+    if ((rc = dmalloc((char**)&src, (size_t)bytes))) return rc;
+    if ((rc = dmalloc((char**)&dst, (size_t)bytes))) { dfree(src); return rc; }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    rc = MHIT_ERR_HIP;
+    if (hipMemsetAsync(src, 0, (size_t)bytes, c->stream) == hipSuccess &&
+        hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+        launch_copy_probe(c, src, dst, (size_t)bytes) == MHIT_OK &&   // warm-up
+        hipEventRecord(e0, c->stream) == hipSuccess) {
+        bool ok = true;
+        for (int t = 0; t < iters && ok; t++) ok = launch_copy_probe(c, src, dst, (size_t)bytes) == MHIT_OK;
+        float ms = 0.0f;
+        if (ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+            hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            *gbps = 2.0 * (double)bytes * iters / (ms * 1e-3) / 1e9;
+            rc = MHIT_OK;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    dfree(src);
+    dfree(dst);
+    if (rc) set_error("copy probe failed");
+    return rc;
 }
 
 int mhit_profile_read(mhit_ctx* c, mhit_profile* out) {

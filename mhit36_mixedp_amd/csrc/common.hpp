@@ -39,6 +39,16 @@ struct DevScalars {
     int pad_;
 };
 
+// Launch-shape choices that do not change results (z-chunk lengths of the
+// marching passes, XCD-aware tile order).  Defaults are the measured best
+// at 512^3; MHIT_TUNE="ns_mz=64,ac_mz=64,xcd=1,sort=10" overrides them.
+struct Tuning {
+    int ns_mz = 32;       // planes per workgroup, NS pass
+    int ac_mz = 64;       // planes per workgroup, ACDI pass
+    int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
+    int sort_every = 10;  // steps between tracer re-orderings (0: never)
+};
+
 // Per-run constants passed by value to kernels.
 struct Grid {
     int n;                // points per direction
@@ -63,6 +73,23 @@ struct CellLaunch {
     dim3 grid, block;
     explicit CellLaunch(int n) : grid(div_up(n, 64), div_up(n, 4), n), block(64, 4, 1) {}
 };
+
+// XCD-aware tile order.  Workgroups are dispatched round-robin over the 8
+// XCDs of the MI355X (linear id mod 8), each with its own L2.  This remap
+// gives XCD x the contiguous range [x T/8, (x+1) T/8) of tiles in x-fastest
+// order, so the workgroups resident on one XCD at a time cover an x-y patch
+// of neighbouring tiles whose halo rows and columns then come from that
+// XCD's L2.  Identity when the tile count is not a multiple of 8.
+constexpr int kXcds = 8;
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, int enable = 1) {
+    const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+    const int total = nx * ny * nz;
+    int L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    if (enable && total % kXcds == 0) L = (L % kXcds) * (total / kXcds) + L / kXcds;
+    bx = L % nx;
+    by = (L / nx) % ny;
+    bz = L / (nx * ny);
+}
 
 // Block-level reduction helpers (wavefront size 64 on CDNA).
 __device__ __forceinline__ double wave_sum(double v) {

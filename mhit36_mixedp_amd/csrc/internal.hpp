@@ -22,6 +22,8 @@ struct mhit_ctx {
     double *phi = nullptr, *rhsphi = nullptr, *rhsphi_o = nullptr;
     double *phi_next = nullptr;   // ping-pong target of the fused ACDI pass
     bool march = true;            // fused z-marching stencil passes (default)
+    mhit::Tuning tune{};          // launch shapes (MHIT_TUNE)
+    long long steps_since_sort = 0;
     double *psi = nullptr, *tanh_psi = nullptr;
     double *normx = nullptr, *normy = nullptr, *normz = nullptr;
     double *fxst = nullptr, *fyst = nullptr, *fzst = nullptr;
@@ -44,6 +46,14 @@ struct mhit_ctx {
     double *ptag = nullptr, *px = nullptr, *py = nullptr, *pz = nullptr;
     double *pu = nullptr, *pv = nullptr, *pw = nullptr;
     double *pu1 = nullptr, *pv1 = nullptr, *pw1 = nullptr;
+    // cell-order sort of the tracers: keys, permutation, the other half of
+    // the double-buffered columns and the radix-sort workspace
+    unsigned* skey[2] = {nullptr, nullptr};
+    int* sidx[2] = {nullptr, nullptr};
+    double* alt[7] = {};
+    void* sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int64_t sort_cap = 0;
     // host copy of scalars after the last sync
     mhit::DevScalars S_host{};
     // per-stage event profiling (mhit_profile_enable)
@@ -59,7 +69,8 @@ namespace mhit {
 
 // Stages of one timestep, in launch order.
 enum Stage {
-    ST_TRACERS, ST_ACDI_PSI, ST_ACDI_RHS, ST_ACDI_NORM, ST_ACDI_UPDATE, ST_ACDI, ST_CSF, ST_NS,
+    ST_TRACER_SORT, ST_TRACERS, ST_ACDI_PSI, ST_ACDI_RHS, ST_ACDI_NORM, ST_ACDI_UPDATE, ST_ACDI,
+    ST_CSF, ST_NS,
     ST_FFT_X_FWD, ST_FFT_Y_FWD, ST_FFT_Z, ST_FFT_Y_INV, ST_FFT_X_INV, ST_CORRECT, ST_FINALIZE,
     ST_COUNT
 };
@@ -92,6 +103,8 @@ int launch_correct(mhit_ctx* c);
 int launch_finalize(mhit_ctx* c);
 int launch_diag(mhit_ctx* c, double* host_out /* 8 values */);
 int correct_block_count(int n);
+int diag_block_count(int n);
+int launch_copy_probe(mhit_ctx* c, const void* src, void* dst, size_t bytes);
 
 // march.hip
 int launch_acdi_march(mhit_ctx* c);
@@ -99,6 +112,8 @@ int launch_ns_march(mhit_ctx* c);
 
 // tracers.hip
 int launch_tracers(mhit_ctx* c);
+int sort_tracers(mhit_ctx* c);          // stable cell-order (Morton) permutation
+void free_sort_buffers(mhit_ctx* c);
 
 // init.hip
 int launch_init_tgv(mhit_ctx* c);

@@ -30,7 +30,7 @@ __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 // =====================================================================
 // NS + CSF + AB2
 // =====================================================================
-constexpr int NS_MX = 64, NS_MY = 8, NS_MZ = 32;
+constexpr int NS_MX = 64, NS_MY = 8;   // z chunk: Tuning::ns_mz
 constexpr int NS_W1 = NS_MX + 2, NS_H1 = NS_MY + 2;   // halo 1 (velocity)
 constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // halo 2 (phi)
 
@@ -41,7 +41,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
                 double* __restrict__ ru_o, double* __restrict__ rv_o, double* __restrict__ rw_o,
                 double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
                 const double* __restrict__ mysin, const double* __restrict__ mycos,
-                Grid g, const DevScalars* __restrict__ S, int zc) {
+                Grid g, const DevScalars* __restrict__ S, int zc, int xcd) {
     constexpr int MX = NS_MX, MY = NS_MY, W1 = NS_W1, H1 = NS_H1, W2 = NS_W2, H2 = NS_H2;
     constexpr int NT = MX * MY;
     __shared__ double su[3][H1 * W1], sv[3][H1 * W1], sw[3][H1 * W1];
@@ -52,8 +52,10 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     if (S->halted) return;
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
-    const int i0 = blockIdx.x * MX, j0 = blockIdx.y * MY;
-    const int z0 = blockIdx.z * zc;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz, xcd);
+    const int i0 = tbx * MX, j0 = tby * MY;
+    const int z0 = tbz * zc;
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
@@ -210,7 +212,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
 // =====================================================================
 // ACDI
 // =====================================================================
-constexpr int AC_MX = 64, AC_MY = 16, AC_MZ = 64;
+constexpr int AC_MX = 64, AC_MY = 16;  // z chunk: Tuning::ac_mz
 constexpr int AC_W1 = AC_MX + 2, AC_H1 = AC_MY + 2;   // phi ring: halo 1
 constexpr int AC_W2 = AC_MX + 4, AC_H2 = AC_MY + 4;   // psi ring: halo 2
 
@@ -218,7 +220,7 @@ __global__ __launch_bounds__(AC_MX * AC_MY)
 void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
                   const double* __restrict__ u, const double* __restrict__ v,
                   const double* __restrict__ w, double* __restrict__ rphi_o,
-                  Grid g, const DevScalars* __restrict__ S, int zc) {
+                  Grid g, const DevScalars* __restrict__ S, int zc, int xcd) {
     constexpr int MX = AC_MX, MY = AC_MY, W1 = AC_W1, H1 = AC_H1, W2 = AC_W2, H2 = AC_H2;
     constexpr int NT = MX * MY;
     __shared__ double spsi[3][H2 * W2];
@@ -229,8 +231,10 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     if (S->halted) return;
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
-    const int i0 = blockIdx.x * MX, j0 = blockIdx.y * MY;
-    const int z0 = blockIdx.z * zc;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz, xcd);
+    const int i0 = tbx * MX, j0 = tby * MY;
+    const int z0 = tbz * zc;
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
@@ -381,16 +385,16 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
 // ------------------------------------------------------------- launchers
 int launch_ns_march(mhit_ctx* c) {
     const int n = c->n;
-    const int zc = n < NS_MZ ? n : NS_MZ;
+    const int zc = n < c->tune.ns_mz ? n : c->tune.ns_mz;
     dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, zc)), block(NS_MX, NS_MY, 1);
     if (c->P.phiflag)
         k_ns_march<true><<<grid, block, 0, c->stream>>>(
             c->u, c->v, c->w, c->phi, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
-            c->mysin, c->mycos, c->g, c->S, zc);
+            c->mysin, c->mycos, c->g, c->S, zc, c->tune.xcd);
     else
         k_ns_march<false><<<grid, block, 0, c->stream>>>(
             c->u, c->v, c->w, nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
-            c->mysin, c->mycos, c->g, c->S, zc);
+            c->mysin, c->mycos, c->g, c->S, zc, c->tune.xcd);
     prof_mark(c, ST_NS);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
@@ -398,10 +402,10 @@ int launch_ns_march(mhit_ctx* c) {
 
 int launch_acdi_march(mhit_ctx* c) {
     const int n = c->n;
-    const int zc = n < AC_MZ ? n : AC_MZ;
+    const int zc = n < c->tune.ac_mz ? n : c->tune.ac_mz;
     dim3 grid(div_up(n, AC_MX), div_up(n, AC_MY), div_up(n, zc)), block(AC_MX, AC_MY, 1);
     k_acdi_march<<<grid, block, 0, c->stream>>>(c->phi_next, c->phi, c->u, c->v, c->w, c->rhsphi_o,
-                                                c->g, c->S, zc);
+                                                c->g, c->S, zc, c->tune.xcd);
     prof_mark(c, ST_ACDI);
     MHIT_HIP_TRY(hipGetLastError());
     std::swap(c->phi, c->phi_next);

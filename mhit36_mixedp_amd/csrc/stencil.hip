@@ -236,52 +236,91 @@ int launch_ns(mhit_ctx* c) {
 // ----------------------------------------------------------- correction
 // C, main.f90:942-962: u = u* - dt/rho (p - p(-1)) dxi with the pressure
 // promoted to FP64, plus per-block sums and maxima of the corrected u,v,w.
-// The grid covers (x, y) once and loops over z planes with stride
-// gridDim.z, so the number of partials is fixed and the reduction order is
-// deterministic.
-constexpr int kCorrZ = 16;
+// A workgroup owns a 64 x 8 tile of columns and marches a chunk of 32 z
+// planes: p(z-1) of its column stays in a register, the plane's p with a
+// one-point halo at -x and -y is staged in LDS, and the next plane is
+// fetched into registers during the update.  The number of partials is
+// fixed by the grid, so the reduction order is deterministic.
+constexpr int kCorrZ = 16;                    // z split of the diagnostics pass
+constexpr int CO_MX = 64, CO_MY = 8, CO_MZ = 32;
+
+static dim3 correct_grid(int n) {
+    const int zc = n < CO_MZ ? n : CO_MZ;
+    return dim3(div_up(n, CO_MX), div_up(n, CO_MY), div_up(n, zc));
+}
 
 template <typename T>
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(CO_MX * CO_MY)
 void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
                const double* __restrict__ ws, const T* __restrict__ p,
                double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-               double* __restrict__ partials, Grid g, const DevScalars* __restrict__ S) {
-    __shared__ double red[6][4];
+               double* __restrict__ partials, Grid g, const DevScalars* __restrict__ S,
+               int zc, int xcd) {
+    constexpr int MX = CO_MX, MY = CO_MY, NT = MX * MY, W = MX + 1, H = MY + 1;
+    constexpr int NP = (W * H + NT - 1) / NT;
+    __shared__ T sp[W * H];                   // x = -1..MX-1, y = -1..MY-1
+    __shared__ double red[6][NT / 64];
     const int n = g.n;
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const bool live = !S->halted && i < n && j < n;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
+    int tbx, tby, tbz;
+    xcd_tile(tbx, tby, tbz, xcd);
+    const int i0 = tbx * MX, j0 = tby * MY, z0 = tbz * zc;
+    const int z1 = min(z0 + zc, n);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool halted = S->halted != 0;
+    const bool live = !halted && i < n && j < n;
     const double dtr = g.dt / g.rho;
     double su = 0.0, sv = 0.0, sw = 0.0;
     double mu_ = -INFINITY, mv_ = -INFINITY, mw_ = -INFINITY;
-    if (live) {
-        const int im = wrap_m(i, n), jm = wrap_m(j, n);
-        for (int k = blockIdx.z; k < n; k += gridDim.z) {
-            const int km = wrap_m(k, n);
-            const size_t q = idx3(i, j, k, n);
-            const double pc = (double)p[q];
-            const double uu = us[q] - dtr * (pc - (double)p[idx3(im, j, k, n)]) * g.dxi;
-            const double vv = vs[q] - dtr * (pc - (double)p[idx3(i, jm, k, n)]) * g.dxi;
-            const double ww = ws[q] - dtr * (pc - (double)p[idx3(i, j, km, n)]) * g.dxi;
-            u[q] = uu; v[q] = vv; w[q] = ww;
-            su += uu; sv += vv; sw += ww;
-            mu_ = fmax(mu_, uu); mv_ = fmax(mv_, vv); mw_ = fmax(mw_, ww);
+    if (!halted) {
+        T pf[NP];
+        auto fetch = [&](int k) {
+#pragma unroll
+            for (int t = 0; t < NP; t++) {
+                const int e = tid + t * NT;
+                if (e < W * H) {
+                    const int lx = e % W, ly = e / W;
+                    pf[t] = p[idx3((i0 + lx - 1 + n) % n, (j0 + ly - 1 + n) % n, k, n)];
+                }
+            }
+        };
+        double pkm = live ? (double)p[idx3(i, j, wrap_m(z0, n), n)] : 0.0;
+        fetch(z0);
+        for (int k = z0; k < z1; k++) {
+#pragma unroll
+            for (int t = 0; t < NP; t++) {
+                const int e = tid + t * NT;
+                if (e < W * H) sp[e] = pf[t];
+            }
+            __syncthreads();
+            if (k + 1 < z1) fetch(k + 1);
+            if (live) {
+                const size_t q = idx3(i, j, k, n);
+                const int c = (ty + 1) * W + tx + 1;
+                const double pc = (double)sp[c];
+                const double uu = us[q] - dtr * (pc - (double)sp[c - 1]) * g.dxi;
+                const double vv = vs[q] - dtr * (pc - (double)sp[c - W]) * g.dxi;
+                const double ww = ws[q] - dtr * (pc - pkm) * g.dxi;
+                u[q] = uu; v[q] = vv; w[q] = ww;
+                su += uu; sv += vv; sw += ww;
+                mu_ = fmax(mu_, uu); mv_ = fmax(mv_, vv); mw_ = fmax(mw_, ww);
+                pkm = pc;
+            }
+            __syncthreads();
         }
     }
     su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
     mu_ = wave_max(mu_); mv_ = wave_max(mv_); mw_ = wave_max(mw_);
-    const int lane = threadIdx.x & 63;        // blockDim.x == 64: one wave per y row
-    const int wv = threadIdx.y;
-    if (lane == 0) {
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) {
         red[0][wv] = su; red[1][wv] = sv; red[2][wv] = sw;
         red[3][wv] = mu_; red[4][wv] = mv_; red[5][wv] = mw_;
     }
     __syncthreads();
-    if (threadIdx.y == 0 && threadIdx.x < 6) {
-        const int f = threadIdx.x;
+    if (tid < 6) {
+        const int f = tid;
         double r = red[f][0];
-        for (int t = 1; t < 4; t++) r = (f < 3) ? r + red[f][t] : fmax(r, red[f][t]);
+        for (int t = 1; t < NT / 64; t++) r = (f < 3) ? r + red[f][t] : fmax(r, red[f][t]);
         const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         partials[b * 6 + f] = r;
     }
@@ -289,19 +328,27 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
 
 int launch_correct(mhit_ctx* c) {
     const int n = c->n;
-    dim3 grid(div_up(n, 64), div_up(n, 4), kCorrZ < n ? kCorrZ : n), block(64, 4, 1);
+    const int zc = n < CO_MZ ? n : CO_MZ;
+    dim3 grid = correct_grid(n), block(CO_MX, CO_MY, 1);
     if (c->fp32)
         k_correct<float><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const float*)c->pres,
-                                                       c->u, c->v, c->w, c->partials, c->g, c->S);
+                                                       c->u, c->v, c->w, c->partials, c->g, c->S,
+                                                       zc, c->tune.xcd);
     else
         k_correct<double><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const double*)c->pres,
-                                                        c->u, c->v, c->w, c->partials, c->g, c->S);
+                                                        c->u, c->v, c->w, c->partials, c->g, c->S,
+                                                        zc, c->tune.xcd);
     prof_mark(c, ST_CORRECT);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
 
 int correct_block_count(int n) {
+    const dim3 g = correct_grid(n);
+    return (int)(g.x * g.y * g.z);
+}
+
+int diag_block_count(int n) {
     return div_up(n, 64) * div_up(n, 4) * (kCorrZ < n ? kCorrZ : n);
 }
 
@@ -421,6 +468,27 @@ void k_diag(const double* __restrict__ u, const double* __restrict__ v,
         const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
         partials[b * NF + f] = r;
     }
+}
+
+// Plain streaming copy: 16-byte vectors, four independent loads in flight
+// per thread before the stores, one pass over the buffer.
+__global__ __launch_bounds__(256)
+void k_copy_probe(const double2* __restrict__ src, double2* __restrict__ dst, size_t n2) {
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    double2 r[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (base + t * 256 < n2) r[t] = src[base + t * 256];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (base + t * 256 < n2) dst[base + t * 256] = r[t];
+}
+
+int launch_copy_probe(mhit_ctx* c, const void* src, void* dst, size_t bytes) {
+    const size_t n2 = bytes / sizeof(double2);
+    k_copy_probe<<<div_up((long long)n2, 1024), 256, 0, c->stream>>>((const double2*)src, (double2*)dst, n2);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
 }
 
 int launch_diag(mhit_ctx* c, double* out) {

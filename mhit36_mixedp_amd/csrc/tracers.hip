@@ -8,6 +8,9 @@
 // so it has no counterpart here; tracers keep their order and their tag.
 #include "internal.hpp"
 
+#include <hipcub/hipcub.hpp>
+#include <utility>
+
 namespace mhit {
 
 __device__ __forceinline__ int pmod(long long a, int n) {
@@ -88,6 +91,92 @@ void k_tracers(const double* __restrict__ u, const double* __restrict__ v, const
     z = z - lx * (double)(int)floor(z / lx);
     px[p] = x; py[p] = y; pz[p] = z;
     pu1[p] = up; pv1[p] = vp; pw1[p] = wp;
+}
+
+// ----------------------------------------------------------- cell order
+// Tracers are re-ordered every few steps by the Morton code of their cell
+// (stable radix sort), so that the gathers of consecutive tracers share
+// cache lines.  This is a permutation only: every tracer keeps its tag and
+// its values, and the order is a deterministic function of the positions.
+__device__ __forceinline__ unsigned spread3(unsigned v) {
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(256)
+void k_tracer_keys(const double* __restrict__ px, const double* __restrict__ py,
+                   const double* __restrict__ pz, unsigned* __restrict__ keys, int* __restrict__ idx,
+                   long long np, Grid g) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int n = g.n;
+    const int ci = pmod((long long)floor(px[p] / g.dx), n);
+    const int cj = pmod((long long)floor(py[p] / g.dx), n);
+    const int ck = pmod((long long)floor(pz[p] / g.dx), n);
+    keys[p] = spread3((unsigned)ci) | (spread3((unsigned)cj) << 1) | (spread3((unsigned)ck) << 2);
+    idx[p] = (int)p;
+}
+
+struct Cols7 { double* c[7]; };
+
+__global__ __launch_bounds__(256)
+void k_tracer_permute(const int* __restrict__ perm, Cols7 src, Cols7 dst, long long np) {
+    const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int s = perm[p];
+#pragma unroll
+    for (int t = 0; t < 7; t++) dst.c[t][p] = src.c[t][s];
+}
+
+void free_sort_buffers(mhit_ctx* c) {
+    void* ptrs[] = {c->skey[0], c->skey[1], c->sidx[0], c->sidx[1], c->sort_tmp,
+                    c->alt[0], c->alt[1], c->alt[2], c->alt[3], c->alt[4], c->alt[5], c->alt[6]};
+    for (void* q : ptrs)
+        if (q) (void)hipFree(q);
+    c->skey[0] = c->skey[1] = nullptr;
+    c->sidx[0] = c->sidx[1] = nullptr;
+    c->sort_tmp = nullptr;
+    for (double*& a : c->alt) a = nullptr;
+    c->sort_tmp_bytes = 0;
+    c->sort_cap = 0;
+}
+
+int sort_tracers(mhit_ctx* c) {
+    const long long np = c->np;
+    if (np <= 1) return MHIT_OK;
+    const int bits = 3 * 10;
+    if (np > c->sort_cap) {
+        free_sort_buffers(c);
+        for (int t = 0; t < 2; t++) {
+            MHIT_HIP_TRY(hipMalloc((void**)&c->skey[t], np * sizeof(unsigned)));
+            MHIT_HIP_TRY(hipMalloc((void**)&c->sidx[t], np * sizeof(int)));
+        }
+        for (double*& a : c->alt) MHIT_HIP_TRY(hipMalloc((void**)&a, np * sizeof(double)));
+        size_t tmp = 0;
+        MHIT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->skey[0], c->skey[1], c->sidx[0],
+                                                        c->sidx[1], (int)np, 0, bits, c->stream));
+        MHIT_HIP_TRY(hipMalloc(&c->sort_tmp, tmp));
+        c->sort_tmp_bytes = tmp;
+        c->sort_cap = np;
+    }
+    const int blocks = div_up(np, 256);
+    k_tracer_keys<<<blocks, 256, 0, c->stream>>>(c->px, c->py, c->pz, c->skey[0], c->sidx[0], np, c->g);
+    MHIT_HIP_TRY(hipGetLastError());
+    size_t tmp = c->sort_tmp_bytes;
+    MHIT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->skey[0], c->skey[1], c->sidx[0],
+                                                    c->sidx[1], (int)np, 0, bits, c->stream));
+    double** cols[7] = {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1};
+    Cols7 src, dst;
+    for (int t = 0; t < 7; t++) { src.c[t] = *cols[t]; dst.c[t] = c->alt[t]; }
+    k_tracer_permute<<<blocks, 256, 0, c->stream>>>(c->sidx[1], src, dst, np);
+    MHIT_HIP_TRY(hipGetLastError());
+    for (int t = 0; t < 7; t++) std::swap(*cols[t], c->alt[t]);
+    prof_mark(c, ST_TRACER_SORT);
+    return MHIT_OK;
 }
 
 int launch_tracers(mhit_ctx* c) {

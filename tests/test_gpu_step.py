@@ -45,6 +45,10 @@ def compare(s, o, phi=False, tr=False):
         errs["phi"] = relerr(s.download_field(5), o.get_field(5))
     if tr:
         t = s.download_tracers()
+        # the GPU re-orders tracers by cell; tags 1..N identify them
+        order = np.argsort(t["tag"])
+        assert np.array_equal(t["tag"][order], np.arange(1, t["tag"].size + 1))
+        t = {k: a[order] for k, a in t.items()}
         ox, oy, oz, ou, ov, ow = o.get_tracers()
         errs["xp"] = max(relerr(t["x"], ox), relerr(t["y"], oy), relerr(t["z"], oz))
         errs["up"] = max(relerr(t["up"], ou), relerr(t["vp"], ov), relerr(t["wp"], ow))
