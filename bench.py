@@ -80,6 +80,54 @@ def cpu_baseline(args, fields, tracers):
     return (t1 - t0) * 1e3 / args.cpu_steps, threads
 
 
+ENSEMBLE_SIZES = (64, 96, 128)
+
+
+def ensemble_leg(args):
+    """Ensemble workload: 16 boxes each of 64^3, 96^3 and 128^3 (51,904,512
+    cells), every other box multiphase, each with its own seed, viscosity
+    and drop count.  Returns box-steps/s stepped together (one stream per
+    box) and stepped one box at a time, with the algorithmic HBM rate."""
+    from mhit36_mixedp_amd import Ensemble, default_params
+    params, phase = [], []
+    for i in range(16 * len(ENSEMBLE_SIZES)):
+        n = ENSEMBLE_SIZES[i // 16]
+        ph = i % 2
+        params.append(default_params(nx=n, phiflag=ph, partflag=0, poisson_precision=1, dt=5e-4,
+                                     mu=0.004 + 0.0005 * (i % 9), rho=1.0, sigma=2 * 0.4,
+                                     epsr=1.0))
+        phase.append(ph)
+    cells = sum(int(p.nx) ** 3 for p in params)
+    nbytes = sum(step_bytes_fused(int(p.nx), 0, bool(ph), True) for p, ph in zip(params, phase))
+    with Ensemble(params) as ens:
+        for i, (b, ph) in enumerate(zip(ens.boxes, phase)):
+            b.init_random_velocity(args.seed + i, kmax=4.0, urms=1.0)
+            if ph:
+                b.init_drops(1 + i % 6, 0.4, args.seed + i)
+        ens.step(args.warmup)
+        t0 = time.perf_counter()
+        ens.step(args.ensemble_steps)
+        t1 = time.perf_counter()
+        together = (t1 - t0) / args.ensemble_steps          # s per ensemble step
+        t0 = time.perf_counter()
+        for b in ens.boxes:
+            b.step(args.ensemble_steps)
+        t1 = time.perf_counter()
+        alone = (t1 - t0) / args.ensemble_steps
+        nan = sum(b.diag()["nan_count"] for b in ens.boxes)
+    if nan:
+        raise RuntimeError("non-finite values in the ensemble")
+    nbox = len(params)
+    return {
+        "workload": "16x64^3 + 16x96^3 + 16x128^3 boxes, half multiphase, FP32 Poisson",
+        "boxes": nbox, "cells": cells, "steps": args.ensemble_steps,
+        "box_steps_per_s": round(nbox / together, 1),
+        "ms_per_ensemble_step": round(together * 1e3, 4),
+        "hbm_GBps": round(nbytes / together / 1e9, 1),
+        "one_at_a_time_box_steps_per_s": round(nbox / alone, 1),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +146,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-Poisson form")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-copy-probe", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -145,6 +195,13 @@ def main():
         prof64 = s.profile()
         s.close()
         del s
+
+    # --- ensemble of small boxes ----------------------------------------------------
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble_leg(args)
+        log(f"[bench] ensemble: {ens['box_steps_per_s']} box-steps/s together, "
+            f"{ens['one_at_a_time_box_steps_per_s']} one box at a time, {ens['hbm_GBps']} GB/s")
 
     # --- per-stage table and roofline of the dominant kernel ----------------------
     log(f"[bench] per-stage (FP32 Poisson, {args.profile_steps} profiled steps):")
@@ -205,6 +262,7 @@ def main():
             "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
+        "ensemble": ens,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

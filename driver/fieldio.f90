@@ -10,7 +10,8 @@ module fieldio
   use iso_c_binding
   use mhit_c
   implicit none
-  character(len=3), parameter :: FIELD_TAG(5) = ['u  ', 'v  ', 'w  ', 'p  ', 'phi']
+  character(len=8), parameter :: FIELD_TAG(9) = ['u       ', 'v       ', 'w       ', &
+      'p       ', 'phi     ', 'rhsu_o  ', 'rhsv_o  ', 'rhsw_o  ', 'rhsphi_o']
 
 contains
 
@@ -93,10 +94,13 @@ contains
   end subroutine
 
   ! Tracer positions (not written by the reference, which does not restart
-  ! tracers): raw FP64 tag, x, y, z columns of npart values each.
-  subroutine write_tracers(ctx, t)
+  ! tracers): raw FP64 tag, x, y, z columns of npart values each.  With
+  ! ab2 = .true. the previous-step velocities follow in partab2_%08d.dat
+  ! (up, vp, wp columns) for a restart that continues bit for bit.
+  subroutine write_tracers(ctx, t, ab2)
     type(c_ptr), intent(in) :: ctx
     integer, intent(in) :: t
+    logical, intent(in) :: ab2
     integer(8) :: np
     real(c_double), allocatable :: tag(:), x(:), y(:), z(:), up(:), vp(:), wp(:)
     character(len=256) :: name
@@ -110,6 +114,73 @@ contains
          action='write')
     write(u) tag, x, y, z
     close(u)
+    if (.not. ab2) return
+    write(name,'(a,i8.8,a)') './output/partab2_', t, '.dat'
+    open(newunit=u, file=trim(name), access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u) up, vp, wp
+    close(u)
   end subroutine
+
+  ! Reads part_%08d.dat and partab2_%08d.dat back (the tracer count comes
+  ! from the file size) and restores the full tracer state.
+  subroutine read_tracers(ctx, t)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: t
+    integer(8) :: np, fsize, fsize2
+    real(c_double), allocatable :: tag(:), x(:), y(:), z(:), up(:), vp(:), wp(:)
+    character(len=256) :: name, name2
+    integer :: u
+    write(name,'(a,i8.8,a)') './output/part_', t, '.dat'
+    write(name2,'(a,i8.8,a)') './output/partab2_', t, '.dat'
+    inquire(file=name, size=fsize)
+    inquire(file=name2, size=fsize2)
+    np = fsize / 32_8
+    if (fsize <= 0 .or. mod(fsize, 32_8) /= 0 .or. fsize2 /= np * 24_8) then
+      write(*,*) 'ERROR: tracer restart files ', trim(name), ' / ', trim(name2), &
+          ' are missing or inconsistent'
+      error stop 1
+    end if
+    allocate(tag(np), x(np), y(np), z(np), up(np), vp(np), wp(np))
+    open(newunit=u, file=trim(name), access='stream', form='unformatted', status='old', &
+         action='read')
+    read(u) tag, x, y, z
+    close(u)
+    open(newunit=u, file=trim(name2), access='stream', form='unformatted', status='old', &
+         action='read')
+    read(u) up, vp, wp
+    close(u)
+    call check(mhit_upload_tracer_state(ctx, np, tag, x, y, z, up, vp, wp), &
+        'mhit_upload_tracer_state')
+  end subroutine
+
+  ! AB2 coefficients and gamma of the next step: raw FP64 alpha, beta, gumax.
+  subroutine write_ab2(t, alpha, beta, gumax)
+    integer, intent(in) :: t
+    real(c_double), intent(in) :: alpha, beta, gumax
+    character(len=256) :: name
+    integer :: u
+    write(name,'(a,i8.8,a)') './output/ab2_', t, '.dat'
+    open(newunit=u, file=trim(name), access='stream', form='unformatted', status='replace', &
+         action='write')
+    write(u) alpha, beta, gumax
+    close(u)
+  end subroutine
+
+  logical function read_ab2(t, alpha, beta, gumax)
+    integer, intent(in) :: t
+    real(c_double), intent(out) :: alpha, beta, gumax
+    character(len=256) :: name
+    integer :: u
+    integer(8) :: fsize
+    write(name,'(a,i8.8,a)') './output/ab2_', t, '.dat'
+    inquire(file=name, size=fsize)
+    read_ab2 = fsize == 24_8
+    if (.not. read_ab2) return
+    open(newunit=u, file=trim(name), access='stream', form='unformatted', status='old', &
+         action='read')
+    read(u) alpha, beta, gumax
+    close(u)
+  end function
 
 end module fieldio

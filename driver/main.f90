@@ -7,7 +7,10 @@
 !
 ! What the reference fixes at compile time is given on the command line:
 !   mhit36 [--nx N] [--phi 0|1] [--part 0|1] [--fp64-poisson] [--k2-modified]
-!          [--trilinear] [--seed S] [--quiet]
+!          [--trilinear] [--seed S] [--exact-restart] [--quiet]
+! --exact-restart also dumps the AB2 history (rhs*_o fields, tracer
+! velocities, alpha/beta/gumax) and, on restart, reads it back so that the
+! run continues bit for bit instead of restarting with an Euler step.
 ! (defaults: nx = 256 as multi/module.f90:2, phiflag = partflag = 0 as
 ! multi/main.f90:55-57, FP32 Poisson as the mixedp branch).
 program mhit36
@@ -19,7 +22,9 @@ program mhit36
   type(mhit_params) :: p
   type(mhit_diag) :: d
   type(c_ptr) :: ctx
-  integer :: n, t, tstart, rc, verbose
+  integer :: n, t, tstart, rc, verbose, f
+  logical :: exact
+  real(c_double) :: alpha, beta, gumax
   integer(8) :: seed, count0, count1, rate, countt
   real(c_double), allocatable :: buf(:,:,:)
   character(len=64) :: arg
@@ -28,6 +33,7 @@ program mhit36
   call read_input_inp('input.inp', p)
   seed = 1_8
   verbose = 1
+  exact = .false.
   call parse_args()
   n = p%nx
 
@@ -91,8 +97,26 @@ program mhit36
     end if
   end if
 
+  ! AB2 history of an exact restart (no reference counterpart)
+  if (p%restart /= 0 .and. exact) then
+    write(*,*) 'Exact restart: AB2 history from output folder, iteration:', p%tstart
+    do f = FIELD_RHSU_O, FIELD_RHSW_O
+      call read_field(ctx, f, trim(field_name('./output', f, p%tstart)), n, buf)
+    end do
+    if (p%phiflag == 1) call read_field(ctx, FIELD_RHSPHI_O, &
+        trim(field_name('./output', FIELD_RHSPHI_O, p%tstart)), n, buf)
+    if (.not. read_ab2(p%tstart, alpha, beta, gumax)) then
+      write(*,*) 'ERROR: ./output/ab2 file of iteration', p%tstart, 'missing'
+      error stop 1
+    end if
+    call check(mhit_set_ab2_state(ctx, alpha, beta, gumax), 'mhit_set_ab2_state')
+  end if
+
   ! tracers (main.f90:399-415): uniform random positions
-  if (p%partflag == 1 .and. p%npart > 0) then
+  if (p%partflag == 1 .and. p%restart /= 0 .and. exact) then
+    write(*,*) 'Tracers from output folder, iteration:', p%tstart
+    call read_tracers(ctx, p%tstart)
+  else if (p%partflag == 1 .and. p%npart > 0) then
     write(*,*) 'Random Position whole Domain, tracers:', p%npart
     call check(mhit_init_tracers_uniform(ctx, p%npart, seed), 'mhit_init_tracers_uniform')
   end if
@@ -145,12 +169,19 @@ contains
 
   subroutine dump_fields(tt)
     integer, intent(in) :: tt
-    integer :: f
-    do f = FIELD_U, FIELD_P
-      call write_field(ctx, f, tt, n, buf)
+    integer :: ff
+    do ff = FIELD_U, FIELD_P
+      call write_field(ctx, ff, tt, n, buf)
     end do
     if (p%phiflag == 1) call write_field(ctx, FIELD_PHI, tt, n, buf)
-    if (p%partflag == 1) call write_tracers(ctx, tt)
+    if (p%partflag == 1) call write_tracers(ctx, tt, exact)
+    if (.not. exact) return
+    do ff = FIELD_RHSU_O, FIELD_RHSW_O
+      call write_field(ctx, ff, tt, n, buf)
+    end do
+    if (p%phiflag == 1) call write_field(ctx, FIELD_RHSPHI_O, tt, n, buf)
+    call check(mhit_diag_compute(ctx, d), 'mhit_diag_compute')
+    call write_ab2(tt, d%alpha, d%beta, d%gumax)
   end subroutine
 
   subroutine parse_args()
@@ -174,6 +205,8 @@ contains
         p%k2_mode = MHIT_K2_MODIFIED
       case ('--trilinear')
         p%interp_mode = MHIT_INTERP_TRILINEAR
+      case ('--exact-restart')
+        exact = .true.
       case ('--quiet')
         verbose = 0
       case default

@@ -15,6 +15,9 @@ module mhit_c
   integer(c_int), parameter :: MHIT_INTERP_REFERENCE = 0, MHIT_INTERP_TRILINEAR = 1
   ! field numbers of writefield/readfield (multi/readwrite.f90:21-25)
   integer(c_int), parameter :: FIELD_U = 1, FIELD_V = 2, FIELD_W = 3, FIELD_P = 4, FIELD_PHI = 5
+  ! AB2 history for a restart that continues bit for bit (no reference counterpart)
+  integer(c_int), parameter :: FIELD_RHSU_O = 6, FIELD_RHSV_O = 7, FIELD_RHSW_O = 8, &
+                               FIELD_RHSPHI_O = 9
 
   ! input.inp (multi/input.inp:1-18) plus the reference's compile-time switches
   type, bind(c) :: mhit_params
@@ -63,6 +66,19 @@ module mhit_c
       type(c_ptr), value :: ctx
       integer(c_int64_t), value :: n
       real(c_double), intent(in) :: x(*), y(*), z(*)
+    end function
+    integer(c_int) function mhit_set_ab2_state(ctx, alpha, beta, gumax) &
+        bind(c, name='mhit_set_ab2_state')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: alpha, beta, gumax
+    end function
+    integer(c_int) function mhit_upload_tracer_state(ctx, n, tag, x, y, z, up, vp, wp) &
+        bind(c, name='mhit_upload_tracer_state')
+      import :: c_int, c_ptr, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: tag(*), x(*), y(*), z(*), up(*), vp(*), wp(*)
     end function
     integer(c_int) function mhit_download_tracers(ctx, tag, x, y, z, up, vp, wp) &
         bind(c, name='mhit_download_tracers')

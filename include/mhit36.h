@@ -58,6 +58,14 @@ extern "C" {
 #define MHIT_FIELD_W   3
 #define MHIT_FIELD_P   4
 #define MHIT_FIELD_PHI 5
+/* AB2 history (previous-step right-hand sides), for a restart that
+ * continues bit for bit.  No counterpart in the reference, which restarts
+ * with an Euler step (multi/main.f90:440-441) and keeps these only in
+ * module.f90:58-60, 69.                                                   */
+#define MHIT_FIELD_RHSU_O   6
+#define MHIT_FIELD_RHSV_O   7
+#define MHIT_FIELD_RHSW_O   8
+#define MHIT_FIELD_RHSPHI_O 9
 
 /*
  * Run parameters.  The first block is input.inp, in its order
@@ -121,7 +129,7 @@ int mhit_params_default(mhit_params* p);
 int mhit_create(const mhit_params* p, mhit_ctx** out);
 int mhit_destroy(mhit_ctx* ctx);
 
-/* Copy an interior n^3 FP64 field in/out (fieldn 1..5, see MHIT_FIELD_*).
+/* Copy an interior n^3 FP64 field in/out (fieldn 1..9, see MHIT_FIELD_*).
  * Stand-in for readfield/readfield_restart/writefield
  * (multi/readwrite.f90:1-318).  Downloading U/V/W returns the velocity
  * after the mean removal of main.f90:977-981.  P is the pressure of the
@@ -135,6 +143,14 @@ int mhit_download_field(mhit_ctx* ctx, int fieldn, double* host);
  * (part(:,Itag)) so that callers can match particles after reordering.    */
 int mhit_upload_tracers(mhit_ctx* ctx, int64_t n, const double* x,
                         const double* y, const double* z);
+/* Full tracer state (tag, position, previous-step velocity) as returned by
+ * mhit_download_tracers, for a restart that continues bit for bit.  The
+ * reference's part(:, Itag..) columns (particles.f90:11-22) are what it
+ * would persist; it re-seeds tracers instead (main.f90:399-415).          */
+int mhit_upload_tracer_state(mhit_ctx* ctx, int64_t n, const double* tag,
+                             const double* x, const double* y, const double* z,
+                             const double* up, const double* vp,
+                             const double* wp);
 int mhit_download_tracers(mhit_ctx* ctx, double* tag, double* x, double* y,
                           double* z, double* up, double* vp, double* wp);
 int64_t mhit_tracer_count(const mhit_ctx* ctx);
@@ -185,6 +201,21 @@ int mhit_poisson_solve(mhit_ctx* ctx, const double* rhs, double* p);
 int mhit_fft_r2c(mhit_ctx* ctx, const double* in, double* out);
 /* Inverse unnormalised 3-D C2R FFT (e^{+i}) of the same layout.           */
 int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
+
+/* ---- ensembles of independent boxes ----------------------------------- */
+/* nbox independent MHIT36 boxes (sizes, physics switches and parameters
+ * may differ per box) advanced together on one GPU.  The reference runs one
+ * box per job; this has no reference counterpart (BASELINE ensemble
+ * configuration).  Each box is a full context: initialise, upload and
+ * download through mhit_ensemble_box().  mhit_ensemble_step advances every
+ * box nsteps steps, boxes running concurrently on their own streams;
+ * returns MHIT_ERR_CFL if any box stopped on cou > 7 (that box stays
+ * frozen, the others keep stepping).                                     */
+typedef struct mhit_ensemble mhit_ensemble;
+int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble** out);
+int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);
+int mhit_ensemble_step(mhit_ensemble* ens, int nsteps);
+int mhit_ensemble_destroy(mhit_ensemble* ens);
 
 /* ---- per-stage timing ------------------------------------------------ */
 /* With profiling on, mhit_step records a HIP event after every stage of

@@ -9,4 +9,4 @@ from ._abi import (  # noqa: F401
     K2_CONTINUOUS, K2_MODIFIED, POISSON_FP32, POISSON_FP64, Diag, Params, default_params,
     load_library,
 )
-from .api import CFLStop, MhitError, Solver, drop_centers, version  # noqa: F401
+from .api import CFLStop, Ensemble, MhitError, Solver, drop_centers, version  # noqa: F401

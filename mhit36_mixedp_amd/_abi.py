@@ -24,7 +24,10 @@ INTERP_REFERENCE = 0
 INTERP_TRILINEAR = 1
 
 FIELD_U, FIELD_V, FIELD_W, FIELD_P, FIELD_PHI = 1, 2, 3, 4, 5
-FIELD_NAMES = {FIELD_U: "u", FIELD_V: "v", FIELD_W: "w", FIELD_P: "p", FIELD_PHI: "phi"}
+FIELD_RHSU_O, FIELD_RHSV_O, FIELD_RHSW_O, FIELD_RHSPHI_O = 6, 7, 8, 9
+FIELD_NAMES = {FIELD_U: "u", FIELD_V: "v", FIELD_W: "w", FIELD_P: "p", FIELD_PHI: "phi",
+               FIELD_RHSU_O: "rhsu_o", FIELD_RHSV_O: "rhsv_o", FIELD_RHSW_O: "rhsw_o",
+               FIELD_RHSPHI_O: "rhsphi_o"}
 
 
 class Params(ctypes.Structure):
@@ -145,6 +148,7 @@ EXPORTS = [
     ("mhit_upload_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("mhit_download_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("mhit_upload_tracers", ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, _dp]),
+    ("mhit_upload_tracer_state", ctypes.c_int, [_vp, ctypes.c_int64] + [_dp] * 7),
     ("mhit_download_tracers", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     ("mhit_tracer_count", ctypes.c_int64, [_vp]),
     ("mhit_step", ctypes.c_int, [_vp, ctypes.c_int]),
@@ -164,6 +168,10 @@ EXPORTS = [
     ("mhit_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("mhit_profile_read", ctypes.c_int, [_vp, _P(Profile)]),
     ("mhit_copy_probe", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, _dp]),
+    ("mhit_ensemble_create", ctypes.c_int, [ctypes.c_int32, _P(Params), _P(_vp)]),
+    ("mhit_ensemble_box", ctypes.c_int, [_vp, ctypes.c_int32, _P(_vp)]),
+    ("mhit_ensemble_step", ctypes.c_int, [_vp, ctypes.c_int]),
+    ("mhit_ensemble_destroy", ctypes.c_int, [_vp]),
     ("mhit_version", ctypes.c_char_p, []),
 ]
 

@@ -57,13 +57,17 @@ def drop_centers(ndrops: int, radius: float, min_gap: float, seed: int) -> np.nd
 class Solver:
     """One MHIT36 box on one GPU."""
 
-    def __init__(self, params: Params | None = None, **overrides):
+    def __init__(self, params: Params | None = None, _handle=None, **overrides):
         self.lib = _abi.load_library()
         self.params = params if params is not None else default_params(**overrides)
         if params is not None and overrides:
             for k, v in overrides.items():
                 setattr(self.params, k, v)
         self.n = int(self.params.nx)
+        self._owner = _handle is None
+        if _handle is not None:          # a box owned by an Ensemble
+            self._h = _handle
+            return
         h = ctypes.c_void_p()
         _check(self.lib.mhit_create(ctypes.byref(self.params), ctypes.byref(h)), "mhit_create")
         self._h = h
@@ -71,7 +75,8 @@ class Solver:
     # context management ---------------------------------------------------
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
-            self.lib.mhit_destroy(self._h)
+            if self._owner:
+                self.lib.mhit_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __enter__(self):
@@ -118,6 +123,37 @@ class Solver:
                                                          ("tag", "x", "y", "z", "up", "vp", "wp")]),
                "mhit_download_tracers")
         return cols
+
+    def upload_tracer_state(self, cols: dict):
+        """Full tracer state as returned by ``download_tracers``."""
+        keys = ("tag", "x", "y", "z", "up", "vp", "wp")
+        a = [np.ascontiguousarray(cols[k], dtype=np.float64) for k in keys]
+        _check(self.lib.mhit_upload_tracer_state(self._h, a[0].size, *[_ptr(x) for x in a]),
+               "mhit_upload_tracer_state")
+
+    # checkpoint ---------------------------------------------------------------
+    def checkpoint(self) -> dict:
+        """Everything the next step reads: velocity, phase field, the AB2
+        history (rhs*_o, tracer velocities) and alpha/beta/gumax.  Restoring
+        it into a fresh context continues the run bit for bit; the reference
+        restarts with an Euler step instead (multi/main.f90:440-441)."""
+        fields = [_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W,
+                  _abi.FIELD_RHSU_O, _abi.FIELD_RHSV_O, _abi.FIELD_RHSW_O]
+        if self.params.phiflag:
+            fields += [_abi.FIELD_PHI, _abi.FIELD_RHSPHI_O]
+        d = self.diag()
+        state = {"fields": {f: self.download_field(f) for f in fields},
+                 "alpha": d["alpha"], "beta": d["beta"], "gumax": d["gumax"]}
+        if self.lib.mhit_tracer_count(self._h) > 0:
+            state["tracers"] = self.download_tracers()
+        return state
+
+    def restore(self, state: dict):
+        for f, a in state["fields"].items():
+            self.upload_field(f, a)
+        self.set_ab2_state(state["alpha"], state["beta"], state["gumax"])
+        if "tracers" in state:
+            self.upload_tracer_state(state["tracers"])
 
     # time stepping ----------------------------------------------------------
     def step(self, nsteps: int = 1):
@@ -191,3 +227,46 @@ class Solver:
         out = np.empty((n, n, n))
         _check(self.lib.mhit_fft_c2r(self._h, _ptr(buf), _ptr(out)), "mhit_fft_c2r")
         return out
+
+
+class Ensemble:
+    """Independent boxes advanced together on one GPU (mhit_ensemble_*).
+
+    ``boxes[i]`` is a Solver view of box i (initialise, upload, download,
+    diagnose through it); ``step(n)`` advances every box n steps with the
+    boxes running concurrently."""
+
+    def __init__(self, params_list):
+        self.lib = _abi.load_library()
+        arr = (Params * len(params_list))(*params_list)
+        h = ctypes.c_void_p()
+        _check(self.lib.mhit_ensemble_create(len(params_list), arr, ctypes.byref(h)),
+               "mhit_ensemble_create")
+        self._h = h
+        self.boxes = []
+        for i, p in enumerate(params_list):
+            b = ctypes.c_void_p()
+            _check(self.lib.mhit_ensemble_box(h, i, ctypes.byref(b)), "mhit_ensemble_box")
+            self.boxes.append(Solver(p, _handle=b))
+
+    def step(self, nsteps: int = 1):
+        _check(self.lib.mhit_ensemble_step(self._h, int(nsteps)), "mhit_ensemble_step")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            for b in self.boxes:
+                b.close()
+            self.lib.mhit_ensemble_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

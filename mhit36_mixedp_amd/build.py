@@ -19,7 +19,8 @@ BUILD = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libmhit36.so")
 ARCH = os.environ.get("MHIT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.hip", "poisson.hip", "stencil.hip", "march.hip", "tracers.hip", "init.hip"]
+SOURCES = ["api.hip", "poisson.hip", "stencil.hip", "march.hip", "tracers.hip", "init.hip",
+           "ensemble.hip"]
 
 CXXFLAGS = [
     f"--offload-arch={ARCH}",

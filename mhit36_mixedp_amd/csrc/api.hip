@@ -97,6 +97,10 @@ static double* state_field(mhit_ctx* c, int fieldn) {
         case MHIT_FIELD_V: return c->v;
         case MHIT_FIELD_W: return c->w;
         case MHIT_FIELD_PHI: return c->phi;
+        case MHIT_FIELD_RHSU_O: return c->rhsu_o;
+        case MHIT_FIELD_RHSV_O: return c->rhsv_o;
+        case MHIT_FIELD_RHSW_O: return c->rhsw_o;
+        case MHIT_FIELD_RHSPHI_O: return c->rhsphi_o;
         default: return nullptr;
     }
 }
@@ -136,6 +140,65 @@ double stage_bytes(const mhit_ctx* c, int s) {
     }
 }
 
+// the kernels of one timestep, main.f90:456-1007 (the tracer re-ordering
+// is not part of it)
+static int enqueue_step_kernels(mhit_ctx* c) {
+    int rc;
+    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
+    if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
+    if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
+    if ((rc = poisson_project(c))) return rc;                                // 833-923
+    if ((rc = launch_correct(c))) return rc;                                 // 942-962
+    if ((rc = launch_finalize(c))) return rc;                                // 964-1007
+    return MHIT_OK;
+}
+
+void clear_graphs(mhit_ctx* c) {
+    for (auto& gr : c->graphs) {
+        if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+        gr = mhit_ctx::StepGraph{};
+    }
+}
+
+// One timestep as a replay of a captured graph.  The graph is captured on
+// first use for the current buffers (the fused ACDI pass alternates phi
+// and phi_next; a tracer re-ordering swaps the tracer columns).  All
+// per-step scalars live in device memory, so replays need no host update.
+static int graph_step(mhit_ctx* c) {
+    mhit_ctx::StepGraph* slot = nullptr;
+    for (auto& gr : c->graphs)
+        if (gr.exec && gr.phi_in == c->phi && gr.tag == c->ptag) { slot = &gr; break; }
+    if (!slot) {
+        for (auto& gr : c->graphs)
+            if (!gr.exec) { slot = &gr; break; }
+        if (!slot) {            // all four in use: start over
+            clear_graphs(c);
+            slot = &c->graphs[0];
+        }
+        double* phi_in = c->phi;
+        double* tag = c->ptag;
+        hipGraph_t graph = nullptr;
+        MHIT_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = enqueue_step_kernels(c);    // swaps phi/phi_next on the host
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
+        MHIT_HIP_TRY(e);
+        e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        MHIT_HIP_TRY(e);
+        slot->phi_in = phi_in;
+        slot->phi_out = c->phi;
+        slot->tag = tag;
+        // the capture advanced the host-side buffer roles; undo it so the
+        // launch below applies the step once
+        c->phi = phi_in;
+        if (c->phi_next == phi_in) c->phi_next = slot->phi_out;
+    }
+    MHIT_HIP_TRY(hipGraphLaunch(slot->exec, c->stream));
+    if (slot->phi_out != slot->phi_in) std::swap(c->phi, c->phi_next);
+    return MHIT_OK;
+}
+
 // one timestep, main.f90:456-1007
 static int enqueue_step(mhit_ctx* c) {
     int rc;
@@ -149,12 +212,8 @@ static int enqueue_step(mhit_ctx* c) {
         c->steps_since_sort = 0;
     }
     c->steps_since_sort++;
-    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
-    if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
-    if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
-    if ((rc = poisson_project(c))) return rc;                                // 833-923
-    if ((rc = launch_correct(c))) return rc;                                 // 942-962
-    if ((rc = launch_finalize(c))) return rc;                                // 964-1007
+    if (c->use_graph && !c->prof) return graph_step(c);
+    if ((rc = enqueue_step_kernels(c))) return rc;
     if (c->prof) {
         MHIT_HIP_TRY(hipEventSynchronize(c->prof_ev[c->prof_n - 1]));
         for (int e = 1; e < c->prof_n; e++) {
@@ -170,6 +229,7 @@ static int enqueue_step(mhit_ctx* c) {
 
 static int ensure_tracers(mhit_ctx* c, int64_t n) {
     c->steps_since_sort = c->tune.sort_every;   // re-order on the next step
+    clear_graphs(c);                            // tracer count and columns change
     if (n <= c->np_cap) { c->np = n; return MHIT_OK; }
     double** cols[] = {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1};
     for (double** col : cols) { dfree(*col); *col = nullptr; }
@@ -178,6 +238,28 @@ static int ensure_tracers(mhit_ctx* c, int64_t n) {
         if ((rc = dmalloc(col, (size_t)n))) return rc;
     c->np_cap = n;
     c->np = n;
+    return MHIT_OK;
+}
+
+// Enqueue nsteps timesteps on the context's stream without waiting.
+int step_async(mhit_ctx* c, int nsteps) {
+    int rc;
+    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+    for (int s = 0; s < nsteps; s++)
+        if ((rc = enqueue_step(c))) return rc;
+    return MHIT_OK;
+}
+
+// Wait for the queued steps and read the scalars back; MHIT_ERR_CFL if the
+// CFL check stopped the run (main.f90:1006).
+int finish_steps(mhit_ctx* c) {
+    int rc;
+    if ((rc = sync_scalars(c))) return rc;
+    if (c->S_host.halted) {
+        set_error("cou = " + std::to_string(c->S_host.cou) + " > 7 at step " +
+                  std::to_string(c->S_host.step) + " (main.f90:1006)");
+        return MHIT_ERR_CFL;
+    }
     return MHIT_OK;
 }
 
@@ -229,6 +311,8 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     // follow the reference's loop split (a cross-check of the fused passes)
     if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
     if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
+    // MHIT_GRAPH=0 launches the kernels of every step one by one
+    if (const char* gr = std::getenv("MHIT_GRAPH")) c->use_graph = std::strcmp(gr, "0") != 0;
     // readinput.f90:39-49
     const double twopi = 8.0 * std::atan(1.0);
     Grid& g = c->g;
@@ -343,6 +427,7 @@ int mhit_destroy(mhit_ctx* c) {
                     c->normz, c->fxst, c->fyst, c->fzst, c->pres, c->spec, c->nyq, c->tw, c->kk,
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
                     c->pu1, c->pv1, c->pw1};
+    clear_graphs(c);
     for (void* p : ptrs) dfree(p);
     free_sort_buffers(c);
     for (hipEvent_t& e : c->prof_ev)
@@ -422,6 +507,23 @@ int mhit_upload_tracers(mhit_ctx* c, int64_t n, const double* x, const double* y
     return MHIT_OK;
 }
 
+int mhit_upload_tracer_state(mhit_ctx* c, int64_t n, const double* tag, const double* x,
+                             const double* y, const double* z, const double* up, const double* vp,
+                             const double* wp) {
+    if (!c || n < 0 || (n > 0 && (!tag || !x || !y || !z || !up || !vp || !wp))) {
+        set_error("bad tracer arguments");
+        return MHIT_ERR_ARG;
+    }
+    int rc;
+    if ((rc = ensure_tracers(c, n))) return rc;
+    if (n == 0) return MHIT_OK;
+    const size_t b = (size_t)n * sizeof(double);
+    struct { double* d; const double* h; } cols[] = {{c->ptag, tag}, {c->px, x}, {c->py, y}, {c->pz, z},
+                                                     {c->pu1, up}, {c->pv1, vp}, {c->pw1, wp}};
+    for (auto& col : cols) MHIT_HIP_TRY(hipMemcpy(col.d, col.h, b, hipMemcpyHostToDevice));
+    return MHIT_OK;
+}
+
 int mhit_download_tracers(mhit_ctx* c, double* tag, double* x, double* y, double* z, double* up,
                           double* vp, double* wp) {
     if (!c) { set_error("null context"); return MHIT_ERR_ARG; }
@@ -440,16 +542,8 @@ int64_t mhit_tracer_count(const mhit_ctx* c) { return c ? c->np : -1; }
 int mhit_step(mhit_ctx* c, int nsteps) {
     if (!c || nsteps < 0) { set_error("bad arguments"); return MHIT_ERR_ARG; }
     int rc;
-    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
-    for (int s = 0; s < nsteps; s++)
-        if ((rc = enqueue_step(c))) return rc;
-    if ((rc = sync_scalars(c))) return rc;
-    if (c->S_host.halted) {
-        set_error("cou = " + std::to_string(c->S_host.cou) + " > 7 at step " +
-                  std::to_string(c->S_host.step) + " (main.f90:1006)");
-        return MHIT_ERR_CFL;
-    }
-    return MHIT_OK;
+    if ((rc = step_async(c, nsteps))) return rc;
+    return finish_steps(c);
 }
 
 int mhit_set_ab2_state(mhit_ctx* c, double alpha, double beta, double gumax) {

@@ -91,6 +91,17 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, int enable =
     bz = L / (nx * ny);
 }
 
+// Workgroup barrier for LDS hand-off only.  __syncthreads() carries a
+// workgroup release fence, which on CDNA also waits for every outstanding
+// global load (vmcnt(0)); the marching passes keep the next plane's global
+// loads in flight across barriers, so they synchronise on LDS traffic
+// alone: wait for this wave's LDS operations, then s_barrier.  The memory
+// clobber stops the compiler from moving memory accesses across it; loads
+// into registers are still waited for before their first use.
+__device__ __forceinline__ void lds_sync() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // Block-level reduction helpers (wavefront size 64 on CDNA).
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -56,6 +56,16 @@ struct mhit_ctx {
     int64_t sort_cap = 0;
     // host copy of scalars after the last sync
     mhit::DevScalars S_host{};
+    // captured timesteps, keyed by the buffers they use (phi ping-pong
+    // parity, tracer columns after a re-ordering)
+    struct StepGraph {
+        hipGraphExec_t exec = nullptr;
+        double* phi_in = nullptr;      // phi before the step
+        double* phi_out = nullptr;     // phi after the step
+        double* tag = nullptr;         // first tracer column
+    };
+    bool use_graph = true;
+    StepGraph graphs[4];
     // per-stage event profiling (mhit_profile_enable)
     bool prof = false;
     hipEvent_t prof_ev[40] = {};

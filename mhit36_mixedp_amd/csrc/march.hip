@@ -30,7 +30,7 @@ __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 // =====================================================================
 // NS + CSF + AB2
 // =====================================================================
-constexpr int NS_MX = 64, NS_MY = 8;   // z chunk: Tuning::ns_mz
+constexpr int NS_MX = 64, NS_MY = 16;  // z chunk: Tuning::ns_mz
 constexpr int NS_W1 = NS_MX + 2, NS_H1 = NS_MY + 2;   // halo 1 (velocity)
 constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // halo 2 (phi)
 
@@ -124,9 +124,9 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     commit(z0 - 1);
     fetch(z0);
     commit(z0);
-    __syncthreads();
+    lds_sync();
     if (PHI) fz_prev = sig6 * chem(z0 - 1, tx, ty) * 0.5 * (P(z0, tx, ty) - P(z0 - 2, tx, ty)) * dxi;
-    __syncthreads();
+    lds_sync();
     fetch(z0 + 1);
 
     for (int k = z0; k < z1; k++) {
@@ -135,7 +135,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
         double rou = 0.0, rov = 0.0, row = 0.0;
         if (live) { rou = ru_o[q]; rov = rv_o[q]; row = rw_o[q]; }
         commit(k + 1);
-        __syncthreads();
+        lds_sync();
         if (PHI) {
             // S1 at plane k: fx on x = -1..MX-1, fy on y = -1..MY-1, fz on the column
             const double c0 = chem(k, tx, ty);
@@ -149,7 +149,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
                 const int a = tid - MY;
                 sfy[a] = sig6 * chem(k, a, -1) * 0.5 * (P(k, a, 0) - P(k, a, -2)) * dxi;
             }
-            __syncthreads();
+            lds_sync();
         }
         if (k + 2 <= z1) fetch(k + 2);   // next plane in flight during the compute
         const int sm = ring3(k - 1), sc = ring3(k), sq = ring3(k + 1);
@@ -205,7 +205,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
             rw_o[q] = rw;
         }
         fz_prev = fz_cur;
-        __syncthreads();
+        lds_sync();
     }
 }
 
@@ -319,20 +319,20 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
         fetch(k);
         commit(k);
     }
-    __syncthreads();
+    lds_sync();
     double fz_m1 = 0.0, fz_0 = 0.0, fz_p1 = 0.0;
     {
         double fx, fy;
         flux(z0 - 1, tx, ty, fx, fy, fz_m1);
     }
-    __syncthreads();
+    lds_sync();
     fetch(z0 + 1);
     commit(z0 + 1);
-    __syncthreads();
+    lds_sync();
     int bf = 0;
     fz_0 = fluxes(z0, bf);
     fetch(z0 + 2);
-    __syncthreads();
+    lds_sync();
 
     double wk = 0.0;
     if (live) wk = w[idx3(i, j, z0, n)] - m2;
@@ -349,7 +349,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
             ro = rphi_o[q];
         }
         commit(k + 2);
-        __syncthreads();
+        lds_sync();
         if (k + 3 <= z1 + 1) fetch(k + 3);   // next plane in flight during the compute
         fz_p1 = fluxes(k + 1, bf ^ 1);
         if (live) {
@@ -378,7 +378,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
         fz_m1 = fz_0;
         fz_0 = fz_p1;
         bf ^= 1;
-        __syncthreads();
+        lds_sync();
     }
 }
 

@@ -45,7 +45,12 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
     const int tid = threadIdx.x;
     const int zp = (z + 1 == N) ? 0 : z + 1;
 
-    for (int e = tid; e < N * P; e += NT) {
+    // compile-time trip counts below let the compiler issue all global loads
+    // of a thread before the LDS writes that wait for them
+#pragma unroll
+    for (int r = 0; r < (N * P + NT - 1) / NT; r++) {
+        const int e = tid + r * NT;
+        if ((N * P) % NT != 0 && e >= N * P) break;
         const int a = e % N, c = e / N;
         double d[2];
 #pragma unroll
@@ -72,7 +77,10 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
     __syncthreads();
     fft_stages<V, N, 1, P, TPC, LD, false>(lds, tid % P, tid / P, tw);
 
-    for (int e = tid; e < H * 2 * P; e += NT) {
+#pragma unroll
+    for (int r = 0; r < (H * 2 * P + NT - 1) / NT; r++) {
+        const int e = tid + r * NT;
+        if ((H * 2 * P) % NT != 0 && e >= H * 2 * P) break;
         const int k = e % H, rr = e / H, cc = rr >> 1;
         const V zk = lds[k * LD + cc];
         const V zm = lds[((N - k) % N) * LD + cc];
@@ -114,7 +122,10 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
     const int z = blockIdx.y;
     const int tid = threadIdx.x;
 
-    for (int e = tid; e < H * P; e += NT) {
+#pragma unroll
+    for (int r = 0; r < (H * P + NT - 1) / NT; r++) {
+        const int e = tid + r * NT;
+        if ((H * P) % NT != 0 && e >= H * P) break;
         const int k = e % H, c = e / H;
         const size_t ra = ((size_t)z * N + y0 + 2 * c) * H;
         const V A = spec[ra + k];
@@ -136,7 +147,10 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
     __syncthreads();
     fft_stages<V, N, 1, P, TPC, LD, true>(lds, tid % P, tid / P, tw);
 
-    for (int e = tid; e < N * 2 * P; e += NT) {
+#pragma unroll
+    for (int r = 0; r < (N * 2 * P + NT - 1) / NT; r++) {
+        const int e = tid + r * NT;
+        if ((N * 2 * P) % NT != 0 && e >= N * 2 * P) break;
         const int x = e % N, rr = e / N;
         const V zz = lds[x * LD + (rr >> 1)];
         pres[idx3(x, y0 + rr, z, N)] = (rr & 1) ? zz.y : zz.x;
@@ -164,10 +178,23 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
     const int tid = threadIdx.x;
     V* __restrict__ g = data + blockIdx.x * bx + blockIdx.y * by;
 
-    for (int e = tid; e < N * C; e += NT) {
-        int a, c;
-        if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
-        lds[a * LD + c] = g[a * sa + c * sc];
+    static_assert((N * C) % NT == 0, "column tile must be a multiple of the workgroup");
+    {
+        V buf[(N * C) / NT];   // all loads of the thread in flight before the LDS writes
+#pragma unroll
+        for (int r = 0; r < (N * C) / NT; r++) {
+            const int e = tid + r * NT;
+            int a, c;
+            if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
+            buf[r] = g[a * sa + c * sc];
+        }
+#pragma unroll
+        for (int r = 0; r < (N * C) / NT; r++) {
+            const int e = tid + r * NT;
+            int a, c;
+            if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
+            lds[a * LD + c] = buf[r];
+        }
     }
     __syncthreads();
     const int c = tid % C, t = tid / C;
@@ -195,7 +222,9 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
         fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, tw);
     }
 
-    for (int e = tid; e < N * C; e += NT) {
+#pragma unroll
+    for (int r = 0; r < (N * C) / NT; r++) {
+        const int e = tid + r * NT;
         int a, cc;
         if (AXIS_CONTIG) { a = e % N; cc = e / N; } else { cc = e % C; a = e / C; }
         g[a * sa + cc * sc] = lds[a * LD + cc];

@@ -287,26 +287,28 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
         double pkm = live ? (double)p[idx3(i, j, wrap_m(z0, n), n)] : 0.0;
         fetch(z0);
         for (int k = z0; k < z1; k++) {
+            const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
+            double usq = 0.0, vsq = 0.0, wsq = 0.0;
+            if (live) { usq = us[q]; vsq = vs[q]; wsq = ws[q]; }   // in flight across the barrier
 #pragma unroll
             for (int t = 0; t < NP; t++) {
                 const int e = tid + t * NT;
                 if (e < W * H) sp[e] = pf[t];
             }
-            __syncthreads();
+            lds_sync();
             if (k + 1 < z1) fetch(k + 1);
             if (live) {
-                const size_t q = idx3(i, j, k, n);
                 const int c = (ty + 1) * W + tx + 1;
                 const double pc = (double)sp[c];
-                const double uu = us[q] - dtr * (pc - (double)sp[c - 1]) * g.dxi;
-                const double vv = vs[q] - dtr * (pc - (double)sp[c - W]) * g.dxi;
-                const double ww = ws[q] - dtr * (pc - pkm) * g.dxi;
+                const double uu = usq - dtr * (pc - (double)sp[c - 1]) * g.dxi;
+                const double vv = vsq - dtr * (pc - (double)sp[c - W]) * g.dxi;
+                const double ww = wsq - dtr * (pc - pkm) * g.dxi;
                 u[q] = uu; v[q] = vv; w[q] = ww;
                 su += uu; sv += vv; sw += ww;
                 mu_ = fmax(mu_, uu); mv_ = fmax(mv_, vv); mw_ = fmax(mw_, ww);
                 pkm = pc;
             }
-            __syncthreads();
+            lds_sync();
         }
     }
     su = wave_sum(su); sv = wave_sum(sv); sw = wave_sum(sw);
