@@ -9,10 +9,13 @@
 //     k = b mod NS, inputs x[b + r N/R] * W_{NS R}^{r k},
 //     DFT_R, outputs y[(b div NS) NS R + k + r NS].
 // Twiddles come from a per-N table W_N^j = exp(-2 pi i j/N) computed on the
-// host in double; the inverse uses the conjugate.  Transforms are
+// host in double and staged in LDS by each workgroup; the inverse uses the
+// conjugate.  Transforms are
 // unnormalised, forward e^{-i}, inverse e^{+i}, as cuFFT's used by the
 // reference (multi/main.f90:853-912).
 #pragma once
+
+#include <type_traits>
 
 #include "common.hpp"
 
@@ -35,13 +38,27 @@ template <bool INV, typename V> __device__ __forceinline__ V c_rot(V a) {
     return r;
 }
 
-// Radix plan: factors of 3 first, then 8s (16 -> 4*4), then a 4 or a 2.
+// Radix plan: factors of 3 first, then radix 8 while a radix-8 stage can
+// still end the plan, a 2 or a 4 in the middle, and 8 last whenever the
+// power-of-two part allows (512 = 8.8.8, 256 = 8.4.8, 128 = 8.2.8,
+// 96 = 3.4.8).  A radix-8 first or last stage with N/8 threads per column
+// takes its inputs from / leaves its outputs in registers in the order of
+// the global loads / stores (see k_fft_cols).
+constexpr int next_radix(int rem) {
+    return (rem % 3 == 0) ? 3
+         : (rem == 8 || rem % 64 == 0) ? 8
+         : (rem == 16) ? 2
+         : (rem == 32) ? 4
+         : (rem == 4) ? 4 : 2;
+}
+constexpr int last_radix(int n) {
+    int ns = 1, r = 1;
+    while (ns < n) { r = next_radix(n / ns); ns *= r; }
+    return r;
+}
 template <int REM> struct NextRadix {
-    static constexpr int value = (REM % 3 == 0) ? 3
-                               : (REM % 8 == 0 && REM != 16) ? 8
-                               : (REM % 4 == 0) ? 4 : 2;
+    static constexpr int value = next_radix(REM);
 };
-
 template <bool INV, typename V> __device__ __forceinline__ void dft2(V* a) {
     V t = a[0];
     a[0] = c_add(t, a[1]);
@@ -140,15 +157,56 @@ __device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restr
     __syncthreads();
 }
 
-template <typename V, int N, int NS, int C, int TPC, int LD, bool INV>
+// Stages from NS (points already combined) up to NS_END, each in LDS.
+template <typename V, int N, int NS, int C, int TPC, int LD, bool INV, int NS_END = N>
 __device__ __forceinline__ void fft_stages(V* lds, int c, int t, const V* __restrict__ tw) {
-    if constexpr (NS < N) {
+    if constexpr (NS < NS_END) {
         constexpr int R = NextRadix<N / NS>::value;
         fft_stage<V, N, R, NS, C, TPC, LD, INV>(lds, c, t, tw);
-        fft_stages<V, N, NS * R, C, TPC, LD, INV>(lds, c, t, tw);
+        fft_stages<V, N, NS * R, C, TPC, LD, INV, NS_END>(lds, c, t, tw);
     }
 }
 
+// Register-direct stages for N/8 threads per column (thread t of column c).
+// First stage (radix 8, NS = 1): inputs x[t + r N/8] in v[r], outputs to
+// LDS at 8t + r.
+template <typename V, int N, int LD, bool INV>
+__device__ __forceinline__ void fft_first8_to_lds(V* lds, int c, int t, V (&v)[8]) {
+    dft8<INV>(v);
+#pragma unroll
+    for (int r = 0; r < 8; r++) lds[(8 * t + r) * LD + c] = v[r];
+}
+// Last stage (radix 8, NS = N/8): inputs from LDS at t + r N/8, outputs
+// X[t + r N/8] in v[r].  The caller places a barrier before LDS is written
+// again.
+template <typename V, int N, int LD, bool INV>
+__device__ __forceinline__ void fft_last8_from_lds(const V* lds, int c, int t, V (&v)[8],
+                                                   const V* __restrict__ tw) {
+    constexpr int NS = N / 8;
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = lds[(t + r * NS) * LD + c];
+    if constexpr (NS > 1) {
+#pragma unroll
+        for (int r = 1; r < 8; r++) {
+            V w = tw[r * t];            // W_N^{r k}, k = t (b = t < NS)
+            if (INV) w.y = -w.y;
+            v[r] = c_mul(v[r], w);
+        }
+    }
+    dft8<INV>(v);
+}
+
+// Copy the twiddle table W_N^j, j < N, into LDS.  Stages read their
+// twiddles from there: the C threads of one butterfly row share an address
+// (broadcast), and no stage waits on a global load.
+template <typename V, int N, int NT>
+__device__ __forceinline__ void stage_twiddles(V* stw, const V* __restrict__ tw, int tid) {
+#pragma unroll
+    for (int r = 0; r < (N + NT - 1) / NT; r++) {
+        const int e = tid + r * NT;
+        if (N % NT == 0 || e < N) stw[e] = tw[e];
+    }
+}
 // ---------------------------------------------------------------- config
 constexpr int pow2_divisor_cap(int x, int cap) {
     int c = 1;

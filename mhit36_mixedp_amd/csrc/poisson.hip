@@ -39,11 +39,13 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
     constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
+    __shared__ V stw[N];
     if (S && S->halted) return;
     const int y0 = blockIdx.x * 2 * P;
     const int z = blockIdx.y;
     const int tid = threadIdx.x;
     const int zp = (z + 1 == N) ? 0 : z + 1;
+    stage_twiddles<V, N, NT>(stw, tw, tid);
 
     // compile-time trip counts below let the compiler issue all global loads
     // of a thread before the LDS writes that wait for them
@@ -75,7 +77,7 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
         lds[a * LD + c] = val;
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, false>(lds, tid % P, tid / P, tw);
+    fft_stages<V, N, 1, P, TPC, LD, false>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (H * 2 * P + NT - 1) / NT; r++) {
@@ -117,10 +119,12 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
     constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
+    __shared__ V stw[N];
     if (S && S->halted) return;
     const int y0 = blockIdx.x * 2 * P;
     const int z = blockIdx.y;
     const int tid = threadIdx.x;
+    stage_twiddles<V, N, NT>(stw, tw, tid);
 
 #pragma unroll
     for (int r = 0; r < (H * P + NT - 1) / NT; r++) {
@@ -145,7 +149,7 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
         }
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, true>(lds, tid % P, tid / P, tw);
+    fft_stages<V, N, 1, P, TPC, LD, true>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (N * 2 * P + NT - 1) / NT; r++) {
@@ -174,11 +178,76 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
     constexpr int LD = AXIS_CONTIG ? C + 1 : C;
     constexpr int NT = C * TPC;
     __shared__ V lds[N * LD];
+    __shared__ V stw[N];
     if (S && S->halted) return;
     const int tid = threadIdx.x;
     V* __restrict__ g = data + blockIdx.x * bx + blockIdx.y * by;
 
     static_assert((N * C) % NT == 0, "column tile must be a multiple of the workgroup");
+    // Register-direct form (strided columns, N/8 threads per column, radix-8
+    // last stage): thread t of column c loads and stores elements t + r N/8,
+    // which are the inputs of its first-stage butterfly and the outputs of
+    // its last-stage butterfly.  The z pass keeps the spectrum in registers
+    // from the last forward stage through the divide into the first inverse
+    // stage.  Half of the LDS round trips and barriers of the general form.
+    constexpr bool DIRECT = !AXIS_CONTIG && N % 8 == 0 && TPC == N / 8 && last_radix(N) == 8;
+    if constexpr (DIRECT) {
+        constexpr bool DIRECT_IN = N >= 64 && next_radix(N) == 8;
+        constexpr int NS1 = DIRECT_IN ? 8 : 1;     // points combined after the first stage
+        constexpr bool INV0 = MODE == 1;
+        const int c = tid % C, t = tid / C;
+        V v[8];
+#pragma unroll
+        for (int r = 0; r < 8; r++) v[r] = g[(size_t)(t + r * TPC) * sa + c * sc];
+        stage_twiddles<V, N, NT>(stw, tw, tid);
+        double k2a[MODE == 2 ? 8 : 1];
+        double kxy = 0.0;
+        int kxi = 0, kyi = 0;
+        if constexpr (MODE == 2) {
+            kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
+            kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+            kxy = kk[kxi] + kk[kyi];
+#pragma unroll
+            for (int r = 0; r < 8; r++) k2a[r] = kk[t + r * TPC];
+        }
+        auto to_lds = [&](auto inv) {
+            if constexpr (DIRECT_IN) {
+                fft_first8_to_lds<V, N, LD, decltype(inv)::value>(lds, c, t, v);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 8; r++) lds[(t + r * TPC) * LD + c] = v[r];
+            }
+        };
+        to_lds(std::integral_constant<bool, INV0>{});
+        __syncthreads();
+        fft_stages<V, N, NS1, C, TPC, LD, INV0, N / 8>(lds, c, t, stw);
+        fft_last8_from_lds<V, N, LD, INV0>(lds, c, t, v, stw);
+        if constexpr (MODE == 2) {
+            // main.f90:883-896: p^ = -r^/(k2 nx ny nz), k2 = kx^2+ky^2+kz^2
+            // in FP64, stored back in the spectrum precision; zero mode 0.
+            // One FP64 reciprocal per element replaces the reference's two
+            // divisions per component (at most 2 ulp apart in FP64).
+            __syncthreads();   // every thread holds its last-stage inputs
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int a = t + r * TPC;
+                if (!NYQ && kxi == 0 && kyi == 0 && a == 0) {
+                    v[r].x = (T)0; v[r].y = (T)0;
+                } else {
+                    const double s = -1.0 / ((kxy + k2a[r]) * ntot);
+                    v[r].x = (T)((double)v[r].x * s);
+                    v[r].y = (T)((double)v[r].y * s);
+                }
+            }
+            to_lds(std::integral_constant<bool, true>{});
+            __syncthreads();
+            fft_stages<V, N, NS1, C, TPC, LD, true, N / 8>(lds, c, t, stw);
+            fft_last8_from_lds<V, N, LD, true>(lds, c, t, v, stw);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; r++) g[(size_t)(t + r * TPC) * sa + c * sc] = v[r];
+        return;
+    }
     {
         V buf[(N * C) / NT];   // all loads of the thread in flight before the LDS writes
 #pragma unroll
@@ -188,6 +257,7 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
             buf[r] = g[a * sa + c * sc];
         }
+        stage_twiddles<V, N, NT>(stw, tw, tid);
 #pragma unroll
         for (int r = 0; r < (N * C) / NT; r++) {
             const int e = tid + r * NT;
@@ -196,30 +266,44 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             lds[a * LD + c] = buf[r];
         }
     }
-    __syncthreads();
     const int c = tid % C, t = tid / C;
-    if (MODE == 0 || MODE == 2) fft_stages<V, N, 1, C, TPC, LD, false>(lds, c, t, tw);
-    else fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, tw);
+    // the z pass: wavenumbers of the thread's elements, loaded with the data
+    constexpr int NA = (N + TPC - 1) / TPC;
+    double k2a[MODE == 2 ? NA : 1];
+    double kxy = 0.0;
+    int kxi = 0, kyi = 0;
+    if constexpr (MODE == 2) {
+        kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
+        kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+        kxy = kk[kxi] + kk[kyi];
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int a = t + q * TPC;
+            k2a[q] = (N % TPC == 0 || a < N) ? kk[a] : 0.0;
+        }
+    }
+    __syncthreads();
+    if (MODE == 0 || MODE == 2) fft_stages<V, N, 1, C, TPC, LD, false>(lds, c, t, stw);
+    else fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, stw);
 
-    if (MODE == 2) {
-        // main.f90:883-896: p^ = -r^/k2/(nx ny nz), k2 = kx^2+ky^2+kz^2
-        // in FP64, stored back in the spectrum precision; zero mode 0.
-        const int kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
-        const int kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
-        const double kxy = kk[kxi] + kk[kyi];
-        for (int a = t; a < N; a += TPC) {
+    if constexpr (MODE == 2) {
+        // the divide as in the register-direct form above
+#pragma unroll
+        for (int q = 0; q < NA; q++) {
+            const int a = t + q * TPC;
+            if (N % TPC != 0 && a >= N) break;
             V val = lds[a * LD + c];
-            const double k2 = kxy + kk[a];
             if (!NYQ && kxi == 0 && kyi == 0 && a == 0) {
                 val.x = (T)0; val.y = (T)0;
             } else {
-                val.x = (T)(-(double)val.x / k2 / ntot);
-                val.y = (T)(-(double)val.y / k2 / ntot);
+                const double s = -1.0 / ((kxy + k2a[q]) * ntot);
+                val.x = (T)((double)val.x * s);
+                val.y = (T)((double)val.y * s);
             }
             lds[a * LD + c] = val;
         }
         __syncthreads();
-        fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, tw);
+        fft_stages<V, N, 1, C, TPC, LD, true>(lds, c, t, stw);
     }
 
 #pragma unroll

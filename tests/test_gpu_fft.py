@@ -63,6 +63,44 @@ def test_poisson_single_mode(n, prec):
     assert relerr(p, ref) < tol
 
 
+def _modes(n):
+    h = n // 2
+    return [(3, -2, 5), (h, 1, -3), (-h + 1, h - 1, 7), (0, 0, h), (1, h, 0),
+            (n // 4, -(n // 4), h - 3)]
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [256, 512])
+def test_poisson_modes_full_size(n, prec):
+    """Six modes at production sizes, including x, y and z Nyquist modes
+    and the highest wavenumbers: rhs = sum cos(m.x + ph) gives
+    p = -sum cos(m.x + ph)/|m|^2 for the continuous k^2 (main.f90:883-896).
+    The tolerance is below the smallest mode's amplitude in p, so a mode
+    sent to the wrong wavenumber fails."""
+    print(f"[poisson modes] n={n} prec={prec}", flush=True)
+    L = 2 * np.pi
+    xs = np.arange(n) * (L / n)
+    rhs = np.zeros((n, n, n))
+    ref = np.zeros((n, n, n))
+    amp = []
+    for j, m in enumerate(_modes(n)):
+        ph = 0.3 + 0.7 * j
+        e = (np.exp(1j * m[2] * xs)[:, None, None] * np.exp(1j * m[1] * xs)[None, :, None]
+             * (np.exp(1j * (m[0] * xs + ph)))[None, None, :]).real
+        k2 = sum(a * a for a in m)
+        rhs += e
+        ref -= e / k2
+        amp.append(1.0 / k2)
+        del e
+    with _solver(n, prec) as s:
+        p = s.poisson_solve(rhs)
+    err = np.abs(p - ref).max()
+    tol = 1e-13 if prec == 0 else 1e-6
+    print(f"   max err {err:.3e}, smallest mode amplitude {min(amp):.3e}", flush=True)
+    assert tol < 0.2 * min(amp)
+    assert err < tol
+
+
 @pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
 @pytest.mark.parametrize("k2", [0, 1], ids=["k2", "k2mod"])
 @pytest.mark.parametrize("n", [32, 64])
