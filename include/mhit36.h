@@ -166,6 +166,23 @@ int mhit_diag_compute(mhit_ctx* ctx, mhit_diag* out);
 /* Set the AB2 / gamma state, e.g. to resume a run (main.f90:440-444).     */
 int mhit_set_ab2_state(mhit_ctx* ctx, double alpha, double beta, double gumax);
 
+/* Launch shapes (no counterpart in the reference, whose start-up tunes
+ * cuDecomp's process grid and backends instead, multi/main.f90:80-136).
+ * None of these changes a result bit.                                     */
+typedef struct mhit_tuning {
+    int32_t ns_mz;        /* z planes per workgroup, fused NS pass          */
+    int32_t ac_mz;        /* z planes per workgroup, fused ACDI pass        */
+    int32_t xcd;          /* 1: XCD-aware workgroup order                   */
+    int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
+    double ms_per_step;   /* mhit_autotune: measured with these settings    */
+} mhit_tuning;
+int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);
+int mhit_set_tuning(mhit_ctx* ctx, const mhit_tuning* t);
+/* Time candidate shapes on the context's own state (`steps` timed steps
+ * per candidate), keep the fastest, and restore the state: the run then
+ * continues exactly as if the call had not been made.                     */
+int mhit_autotune(mhit_ctx* ctx, int steps, mhit_tuning* out);
+
 /* Block until all work queued on the context's stream is done.            */
 int mhit_sync(mhit_ctx* ctx);
 

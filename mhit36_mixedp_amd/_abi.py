@@ -90,6 +90,21 @@ class Diag(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """mhit_tuning: launch shapes (none changes a result bit)."""
+
+    _fields_ = [
+        ("ns_mz", ctypes.c_int32),
+        ("ac_mz", ctypes.c_int32),
+        ("xcd", ctypes.c_int32),
+        ("sort_every", ctypes.c_int32),
+        ("ms_per_step", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 PROF_MAX = 24
 
 
@@ -155,6 +170,9 @@ EXPORTS = [
     ("mhit_diag_compute", ctypes.c_int, [_vp, _P(Diag)]),
     ("mhit_set_ab2_state", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("mhit_sync", ctypes.c_int, [_vp]),
+    ("mhit_get_tuning", ctypes.c_int, [_vp, _P(Tuning)]),
+    ("mhit_set_tuning", ctypes.c_int, [_vp, _P(Tuning)]),
+    ("mhit_autotune", ctypes.c_int, [_vp, ctypes.c_int, _P(Tuning)]),
     ("mhit_last_error", ctypes.c_char_p, []),
     ("mhit_init_taylor_green", ctypes.c_int, [_vp]),
     ("mhit_init_drop", ctypes.c_int, [_vp]),

@@ -170,6 +170,25 @@ class Solver:
         _check(self.lib.mhit_diag_compute(self._h, ctypes.byref(d)), "mhit_diag_compute")
         return d.as_dict()
 
+    def tuning(self) -> dict:
+        t = _abi.Tuning()
+        _check(self.lib.mhit_get_tuning(self._h, ctypes.byref(t)), "mhit_get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        t = _abi.Tuning()
+        _check(self.lib.mhit_get_tuning(self._h, ctypes.byref(t)), "mhit_get_tuning")
+        for k, v in kw.items():
+            setattr(t, k, v)
+        _check(self.lib.mhit_set_tuning(self._h, ctypes.byref(t)), "mhit_set_tuning")
+
+    def autotune(self, steps: int = 5) -> dict:
+        """Pick the fastest launch shapes on the current state; the state is
+        restored afterwards."""
+        t = _abi.Tuning()
+        _check(self.lib.mhit_autotune(self._h, int(steps), ctypes.byref(t)), "mhit_autotune")
+        return t.as_dict()
+
     def profile_enable(self, on: bool = True):
         _check(self.lib.mhit_profile_enable(self._h, int(on)), "mhit_profile_enable")
 

@@ -20,7 +20,7 @@ OUT = os.path.join(HERE, "libmhit36.so")
 ARCH = os.environ.get("MHIT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "poisson.hip", "stencil.hip", "march.hip", "tracers.hip", "init.hip",
-           "ensemble.hip"]
+           "ensemble.hip", "tune.hip"]
 
 CXXFLAGS = [
     f"--offload-arch={ARCH}",

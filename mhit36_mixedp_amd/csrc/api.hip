@@ -154,6 +154,7 @@ static int enqueue_step_kernels(mhit_ctx* c) {
 }
 
 void clear_graphs(mhit_ctx* c) {
+    c->graph_gen++;   // also invalidates the ensemble graphs that hold this box
     for (auto& gr : c->graphs) {
         if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
         gr = mhit_ctx::StepGraph{};
@@ -199,6 +200,25 @@ static int graph_step(mhit_ctx* c) {
     return MHIT_OK;
 }
 
+int sync_scalars_public(mhit_ctx* c) { return sync_scalars(c); }
+int push_scalars_public(mhit_ctx* c) { return push_scalars(c); }
+
+// The part of a timestep that stays outside a captured graph: the tracer
+// re-ordering, every sort_every steps.
+int step_eager_part(mhit_ctx* c) {
+    int rc;
+    if (c->P.partflag && c->np > 1 && c->tune.sort_every > 0 &&
+        c->steps_since_sort >= c->tune.sort_every) {
+        if ((rc = sort_tracers(c))) return rc;
+        c->steps_since_sort = 0;
+    }
+    c->steps_since_sort++;
+    return MHIT_OK;
+}
+
+// The kernels of one timestep on the context's stream (for capture).
+int step_kernels(mhit_ctx* c) { return enqueue_step_kernels(c); }
+
 // one timestep, main.f90:456-1007
 static int enqueue_step(mhit_ctx* c) {
     int rc;
@@ -206,12 +226,7 @@ static int enqueue_step(mhit_ctx* c) {
         c->prof_n = 0;
         prof_mark(c, -1);
     }
-    if (c->P.partflag && c->np > 1 && c->tune.sort_every > 0 &&
-        c->steps_since_sort >= c->tune.sort_every) {
-        if ((rc = sort_tracers(c))) return rc;
-        c->steps_since_sort = 0;
-    }
-    c->steps_since_sort++;
+    if ((rc = step_eager_part(c))) return rc;
     if (c->use_graph && !c->prof) return graph_step(c);
     if ((rc = enqueue_step_kernels(c))) return rc;
     if (c->prof) {

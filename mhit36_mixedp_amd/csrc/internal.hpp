@@ -66,6 +66,7 @@ struct mhit_ctx {
     };
     bool use_graph = true;
     StepGraph graphs[4];
+    unsigned long long graph_gen = 0;   // bumped whenever captured graphs become invalid
     // per-stage event profiling (mhit_profile_enable)
     bool prof = false;
     hipEvent_t prof_ev[40] = {};

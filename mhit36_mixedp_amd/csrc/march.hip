@@ -66,27 +66,37 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
 
     // one plane in flight: global -> registers (fetch), registers -> ring (commit)
     constexpr int NV = (H1 * W1 + NT - 1) / NT, NPH = (H2 * W2 + NT - 1) / NT;
+    const size_t plane = (size_t)n * n;
+    // in-plane offsets of the thread's halo points, fixed for the whole march
+    int voff[NV], poff[PHI ? NPH : 1];
+#pragma unroll
+    for (int t = 0; t < NV; t++) {
+        const int e = tid + t * NT;
+        const int lx = e % W1, ly = e / W1;
+        voff[t] = (e < H1 * W1) ? wrapn(j0 - 1 + ly, n) * n + wrapn(i0 - 1 + lx, n) : 0;
+    }
+    if (PHI) {
+#pragma unroll
+        for (int t = 0; t < NPH; t++) {
+            const int e = tid + t * NT;
+            const int lx = e % W2, ly = e / W2;
+            poff[t] = (e < H2 * W2) ? wrapn(j0 - 2 + ly, n) * n + wrapn(i0 - 2 + lx, n) : 0;
+        }
+    }
     double pu[NV], pv[NV], pw[NV], pp[NPH];
     auto fetch = [&](int k) {
-        const int kk = wrapn(k, n);
+        const size_t base = (size_t)wrapn(k, n) * plane;
 #pragma unroll
         for (int t = 0; t < NV; t++) {
-            const int e = tid + t * NT;
-            if (e < H1 * W1) {
-                const int lx = e % W1, ly = e / W1;
-                const size_t q = idx3(wrapn(i0 - 1 + lx, n), wrapn(j0 - 1 + ly, n), kk, n);
+            if ((H1 * W1) % NT == 0 || tid + t * NT < H1 * W1) {
+                const size_t q = base + voff[t];
                 pu[t] = u[q]; pv[t] = v[q]; pw[t] = w[q];
             }
         }
         if (PHI) {
 #pragma unroll
-            for (int t = 0; t < NPH; t++) {
-                const int e = tid + t * NT;
-                if (e < H2 * W2) {
-                    const int lx = e % W2, ly = e / W2;
-                    pp[t] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
-                }
-            }
+            for (int t = 0; t < NPH; t++)
+                if ((H2 * W2) % NT == 0 || tid + t * NT < H2 * W2) pp[t] = phi[base + poff[t]];
         }
     };
     auto commit = [&](int k) {
@@ -228,7 +238,6 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     __shared__ double sFx[2][MY * (MX + 2)];   // x = -1..MX
     __shared__ double sFy[2][(MY + 2) * MX];   // y = -1..MY
 
-    if (S->halted) return;
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
     int tbx, tby, tbz;
@@ -238,6 +247,13 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
+    if (S->halted) {
+        // a stopped run keeps its state: phi_out becomes a copy of phi, so
+        // the host-side ping-pong of the two buffers stays consistent
+        if (live)
+            for (int k = z0; k < z1; k++) phi_out[idx3(i, j, k, n)] = phi[idx3(i, j, k, n)];
+        return;
+    }
     const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
     const double alpha = S->alpha, beta = S->beta;
     const double gamma = 1.0 * S->gumax;               // main.f90:536
@@ -245,17 +261,21 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
 
     // phi of one plane in flight: global -> registers
     constexpr int NPL = (H2 * W2 + NT - 1) / NT;
+    const size_t plane = (size_t)n * n;
+    // in-plane offsets of the thread's halo points, fixed for the whole march
+    int poff[NPL];
+#pragma unroll
+    for (int t = 0; t < NPL; t++) {
+        const int e = tid + t * NT;
+        const int lx = e % W2, ly = e / W2;
+        poff[t] = (e < H2 * W2) ? wrapn(j0 - 2 + ly, n) * n + wrapn(i0 - 2 + lx, n) : 0;
+    }
     double pf[NPL];
     auto fetch = [&](int k) {
-        const int kk = wrapn(k, n);
+        const double* __restrict__ pl = phi + (size_t)wrapn(k, n) * plane;
 #pragma unroll
-        for (int t = 0; t < NPL; t++) {
-            const int e = tid + t * NT;
-            if (e < H2 * W2) {
-                const int lx = e % W2, ly = e / W2;
-                pf[t] = phi[idx3(wrapn(i0 - 2 + lx, n), wrapn(j0 - 2 + ly, n), kk, n)];
-            }
-        }
+        for (int t = 0; t < NPL; t++)
+            if ((H2 * W2) % NT == 0 || tid + t * NT < H2 * W2) pf[t] = pl[poff[t]];
     };
     // registers -> psi ring (halo 2) and phi ring (halo 1), main.f90:522-534
     auto commit = [&](int k) {
@@ -267,7 +287,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
             const int lx = e % W2, ly = e / W2;
             const double ph = pf[t];
             const double val = fmin(ph, 1.0);
-            spsi[sps][e] = eps * log((val + enm) / (1.0 - val + enm));
+            spsi[sps][e] = eps * ln_ratio(val + enm, 1.0 - val + enm);
             if (lx >= 1 && lx <= MX + 2 && ly >= 1 && ly <= MY + 2)
                 sphi[sph][(ly - 1) * W1 + (lx - 1)] = ph;
         }

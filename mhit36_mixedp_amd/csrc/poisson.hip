@@ -200,15 +200,9 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
 #pragma unroll
         for (int r = 0; r < 8; r++) v[r] = g[(size_t)(t + r * TPC) * sa + c * sc];
         stage_twiddles<V, N, NT>(stw, tw, tid);
-        double k2a[MODE == 2 ? 8 : 1];
-        double kxy = 0.0;
-        int kxi = 0, kyi = 0;
+        __shared__ double skk[MODE == 2 ? N : 1];   // per-axis k^2, staged with the data
         if constexpr (MODE == 2) {
-            kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
-            kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
-            kxy = kk[kxi] + kk[kyi];
-#pragma unroll
-            for (int r = 0; r < 8; r++) k2a[r] = kk[t + r * TPC];
+            for (int e = tid; e < N; e += NT) skk[e] = kk[e];
         }
         auto to_lds = [&](auto inv) {
             if constexpr (DIRECT_IN) {
@@ -228,13 +222,16 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             // One FP64 reciprocal per element replaces the reference's two
             // divisions per component (at most 2 ulp apart in FP64).
             __syncthreads();   // every thread holds its last-stage inputs
+            const int kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
+            const int kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+            const double kxy = skk[kxi] + skk[kyi];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
                 const int a = t + r * TPC;
                 if (!NYQ && kxi == 0 && kyi == 0 && a == 0) {
                     v[r].x = (T)0; v[r].y = (T)0;
                 } else {
-                    const double s = -1.0 / ((kxy + k2a[r]) * ntot);
+                    const double s = -1.0 / ((kxy + skk[a]) * ntot);
                     v[r].x = (T)((double)v[r].x * s);
                     v[r].y = (T)((double)v[r].y * s);
                 }

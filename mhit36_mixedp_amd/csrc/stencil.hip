@@ -273,16 +273,20 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
     double su = 0.0, sv = 0.0, sw = 0.0;
     double mu_ = -INFINITY, mv_ = -INFINITY, mw_ = -INFINITY;
     if (!halted) {
+        // in-plane offsets of the thread's halo points, fixed for the whole march
+        int off[NP];
+#pragma unroll
+        for (int t = 0; t < NP; t++) {
+            const int e = tid + t * NT;
+            const int lx = e % W, ly = e / W;
+            off[t] = (e < W * H) ? ((j0 + ly - 1 + n) % n) * n + (i0 + lx - 1 + n) % n : 0;
+        }
         T pf[NP];
         auto fetch = [&](int k) {
+            const T* __restrict__ pl = p + (size_t)k * n * n;
 #pragma unroll
-            for (int t = 0; t < NP; t++) {
-                const int e = tid + t * NT;
-                if (e < W * H) {
-                    const int lx = e % W, ly = e / W;
-                    pf[t] = p[idx3((i0 + lx - 1 + n) % n, (j0 + ly - 1 + n) % n, k, n)];
-                }
-            }
+            for (int t = 0; t < NP; t++)
+                if ((W * H) % NT == 0 || tid + t * NT < W * H) pf[t] = pl[off[t]];
         };
         double pkm = live ? (double)p[idx3(i, j, wrap_m(z0, n), n)] : 0.0;
         fetch(z0);
@@ -323,7 +327,9 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
         const int f = tid;
         double r = red[f][0];
         for (int t = 1; t < NT / 64; t++) r = (f < 3) ? r + red[f][t] : fmax(r, red[f][t]);
-        const size_t b = ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        // partials in tile order, so the reduction order does not depend on
+        // the workgroup order (xcd_tile)
+        const size_t b = ((size_t)tbz * gridDim.y + tby) * gridDim.x + tbx;
         partials[b * 6 + f] = r;
     }
 }

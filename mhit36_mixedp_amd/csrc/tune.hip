@@ -1,0 +1,202 @@
+// tune.hip -- launch-shape autotuner for one context.
+//
+// The reference tunes its decomposition at start-up with cuDecomp's
+// autotuner (multi/main.f90:80-136: process grid, transpose and halo
+// backends).  On one GPU the free choices are launch shapes instead: the
+// z planes per workgroup of the two fused marching passes and the
+// XCD-aware tile order.  None of them changes a result bit (every cell is
+// computed by the same arithmetic in any shape; the correction's reduction
+// grid is fixed), so the tuner may pick freely.  The tracer re-ordering
+// period (sort_every) is settable but not tuned: its pay-off depends on how
+// far tracers move, which a short trial does not see.
+//
+// mhit_autotune times candidate values one parameter at a time on the
+// context's own state, with the production path (captured graphs), and
+// restores the state after every trial: the run continues as if the tuner
+// had not been called.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace mhit {
+int step_async(mhit_ctx* c, int nsteps);
+int finish_steps(mhit_ctx* c);
+int sync_scalars_public(mhit_ctx* c);
+int push_scalars_public(mhit_ctx* c);
+void clear_graphs(mhit_ctx* c);
+}
+
+using namespace mhit;
+
+namespace {
+
+// Device copy of everything a timestep reads, by role (the ping-pong and
+// re-ordering swap buffers, so restore goes to the current owner of a role).
+struct Snapshot {
+    std::vector<double*> copies;
+    std::vector<size_t> counts;
+    DevScalars S{};
+    long long steps_since_sort = 0;
+};
+
+std::vector<double**> roles(mhit_ctx* c) {
+    std::vector<double**> r = {&c->u, &c->v, &c->w, &c->rhsu_o, &c->rhsv_o, &c->rhsw_o};
+    if (c->P.phiflag) { r.push_back(&c->phi); r.push_back(&c->rhsphi_o); }
+    if (c->np > 0)
+        for (double** col : {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1})
+            r.push_back(col);
+    return r;
+}
+
+size_t role_count(mhit_ctx* c, size_t k) { return k < (c->P.phiflag ? 8u : 6u) ? c->N3 : (size_t)c->np; }
+
+void release(Snapshot& s) {
+    for (double* p : s.copies) (void)hipFree(p);
+    s.copies.clear();
+}
+
+int take(mhit_ctx* c, Snapshot& s) {
+    int rc;
+    if ((rc = sync_scalars_public(c))) return rc;
+    s.S = c->S_host;
+    s.steps_since_sort = c->steps_since_sort;
+    auto r = roles(c);
+    for (size_t k = 0; k < r.size(); k++) {
+        const size_t cnt = role_count(c, k);
+        double* d = nullptr;
+        if (hipMalloc((void**)&d, cnt * sizeof(double)) != hipSuccess) {
+            release(s);
+            set_error("autotune: no device memory for the state snapshot");
+            return MHIT_ERR_NOMEM;
+        }
+        s.copies.push_back(d);
+        s.counts.push_back(cnt);
+        MHIT_HIP_TRY(hipMemcpyAsync(d, *r[k], cnt * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+int restore(mhit_ctx* c, const Snapshot& s) {
+    auto r = roles(c);
+    for (size_t k = 0; k < r.size(); k++)
+        MHIT_HIP_TRY(hipMemcpyAsync(*r[k], s.copies[k], s.counts[k] * sizeof(double),
+                                    hipMemcpyDeviceToDevice, c->stream));
+    c->S_host = s.S;
+    c->steps_since_sort = s.steps_since_sort;
+    int rc;
+    if ((rc = push_scalars_public(c))) return rc;
+    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    return MHIT_OK;
+}
+
+// ms per step of `steps` steps after one untimed step (graph capture).
+int time_steps(mhit_ctx* c, int steps, double* ms) {
+    int rc;
+    clear_graphs(c);   // launch shapes are baked into captured graphs
+    if ((rc = step_async(c, 1))) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    MHIT_HIP_TRY(hipEventCreate(&a));
+    MHIT_HIP_TRY(hipEventCreate(&b));
+    hipError_t e = hipEventRecord(a, c->stream);
+    if (e == hipSuccess) rc = step_async(c, steps);
+    if (e == hipSuccess && rc == MHIT_OK) e = hipEventRecord(b, c->stream);
+    if (e == hipSuccess && rc == MHIT_OK) e = hipEventSynchronize(b);
+    float f = 0.0f;
+    if (e == hipSuccess && rc == MHIT_OK) e = hipEventElapsedTime(&f, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    MHIT_HIP_TRY(e);
+    if ((rc = finish_steps(c))) return rc;   // MHIT_ERR_CFL if a trial stopped the run
+    *ms = (double)f / steps;
+    return MHIT_OK;
+}
+
+void to_public(const Tuning& t, mhit_tuning* out, double ms) {
+    out->ns_mz = t.ns_mz;
+    out->ac_mz = t.ac_mz;
+    out->xcd = t.xcd;
+    out->sort_every = t.sort_every;
+    out->ms_per_step = ms;
+}
+
+bool valid(const mhit_tuning* t) {
+    return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhit_get_tuning(mhit_ctx* c, mhit_tuning* out) {
+    if (!c || !out) { set_error("null argument"); return MHIT_ERR_ARG; }
+    to_public(c->tune, out, 0.0);
+    return MHIT_OK;
+}
+
+int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
+    if (!c || !t || !valid(t)) { set_error("bad tuning"); return MHIT_ERR_ARG; }
+    c->tune.ns_mz = t->ns_mz;
+    c->tune.ac_mz = t->ac_mz;
+    c->tune.xcd = t->xcd;
+    c->tune.sort_every = t->sort_every;
+    clear_graphs(c);
+    return MHIT_OK;
+}
+
+int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
+    if (!c || steps <= 0) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    int rc;
+    if ((rc = sync_scalars_public(c))) return rc;
+    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+    if (c->prof) { set_error("autotune: disable profiling first"); return MHIT_ERR_STATE; }
+    Snapshot snap;
+    if ((rc = take(c, snap))) return rc;
+
+    const int n = c->n;
+    auto zlist = [&](std::initializer_list<int> vals) {
+        std::vector<int> v;
+        for (int z : vals)
+            if (z <= n) v.push_back(z);
+        if (v.empty()) v.push_back(n);
+        return v;
+    };
+    struct Param { int* field; std::vector<int> cand; bool active; };
+    std::vector<Param> params = {
+        {&c->tune.ns_mz, zlist({8, 16, 32, 64, 128}), true},
+        {&c->tune.ac_mz, zlist({16, 32, 64, 128}), c->P.phiflag != 0},
+        {&c->tune.xcd, {0, 1}, true},
+    };
+    double best_ms = 0.0;
+    rc = MHIT_OK;
+    for (Param& p : params) {
+        if (!p.active || rc) continue;
+        int best = *p.field;
+        double bms = 1e300;
+        for (int v : p.cand) {
+            *p.field = v;
+            double ms = 0.0;
+            rc = time_steps(c, steps, &ms);
+            const int rr = restore(c, snap);
+            if (!rc) rc = rr;
+            if (rc) break;
+            if (ms < bms) { bms = ms; best = v; }
+        }
+        *p.field = best;
+        best_ms = bms;
+    }
+    if (!rc) {
+        rc = time_steps(c, steps, &best_ms);
+        const int rr = restore(c, snap);
+        if (!rc) rc = rr;
+    }
+    release(snap);
+    clear_graphs(c);
+    if (rc) return rc;
+    if (out) to_public(c->tune, out, best_ms);
+    return MHIT_OK;
+}
+
+}  // extern "C"

@@ -62,3 +62,23 @@ def test_graph_cfl_stop(monkeypatch):
         with pytest.raises(CFLStop):
             s.step(4)
         assert s.diag()["step"] == 1
+
+
+@pytest.mark.parametrize("graph", ["1", "0"], ids=["graph", "eager"])
+def test_cfl_stop_mid_call_keeps_state(monkeypatch, graph):
+    """Steps queued behind the CFL stop change nothing: the state (phi's
+    ping-pong buffers included) is the one after the stopping step."""
+    from mhit36_mixedp_amd import CFLStop, Solver
+    monkeypatch.setenv("MHIT_GRAPH", graph)
+    kw = dict(nx=16, dt=0.5, phiflag=1, sigma=0.5)
+    with Solver(**kw) as a, Solver(**kw) as b:
+        for s in (a, b):
+            s.init_random_velocity(5, kmax=3.0, urms=20.0)
+            s.init_drops(2, 0.6, 5)
+        with pytest.raises(CFLStop):
+            a.step(4)
+        with pytest.raises(CFLStop):
+            b.step(1)
+        for f in (1, 2, 3, 4, 5):
+            assert np.array_equal(a.download_field(f), b.download_field(f), equal_nan=True), f
+        assert a.diag()["step"] == 1

@@ -1,0 +1,53 @@
+"""The launch-shape autotuner leaves the state as it found it, and no
+launch shape changes a result bit."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _by_tag(t):
+    o = np.argsort(t["tag"])
+    return {k: v[o] for k, v in t.items()}
+
+
+def test_autotune_restores_state_and_results_are_shape_independent():
+    from mhit36_mixedp_amd import Solver
+    print("[autotune] 48^3 multiphase + tracers", flush=True)
+    kw = dict(nx=48, phiflag=1, partflag=1, sigma=0.5, dt=5e-4, poisson_precision=1)
+    with Solver(**kw) as a, Solver(**kw) as b:
+        for s in (a, b):
+            s.init_random_velocity(9, kmax=4.0, urms=1.0)
+            s.init_drops(3, 0.6, 9)
+            s.init_tracers_uniform(4000, 9)
+            s.step(2)
+        t = a.autotune(steps=2)
+        print(t, flush=True)
+        assert t["ms_per_step"] > 0
+        assert t["ns_mz"] in (8, 16, 32) and t["ac_mz"] in (16, 32) and t["xcd"] in (0, 1)
+        assert a.tuning()["ns_mz"] == t["ns_mz"]
+        for f in (1, 2, 3, 5, 6, 7, 8, 9):
+            assert np.array_equal(a.download_field(f), b.download_field(f)), f
+        a.step(3)
+        b.step(3)
+        for f in (1, 2, 3, 4, 5):
+            assert np.array_equal(a.download_field(f), b.download_field(f)), f
+        ta, tb = _by_tag(a.download_tracers()), _by_tag(b.download_tracers())
+        for k in ta:
+            assert np.array_equal(ta[k], tb[k]), k
+        assert a.diag() == b.diag()
+
+
+@pytest.mark.parametrize("shape", [dict(ns_mz=8, ac_mz=16, xcd=1), dict(ns_mz=64, ac_mz=128, xcd=0)])
+def test_set_tuning_is_bitwise_neutral(shape):
+    from mhit36_mixedp_amd import Solver
+    print(f"[tuning] {shape}", flush=True)
+    kw = dict(nx=64, phiflag=1, sigma=0.5, dt=5e-4, poisson_precision=1)
+    with Solver(**kw) as a, Solver(**kw) as b:
+        a.set_tuning(**shape)
+        for s in (a, b):
+            s.init_random_velocity(4, kmax=4.0, urms=1.0)
+            s.init_drops(2, 0.7, 4)
+            s.step(4)
+        for f in (1, 2, 3, 4, 5):
+            assert np.array_equal(a.download_field(f), b.download_field(f)), f
