@@ -59,6 +59,25 @@ def time_steps(s, warmup, steps):
     return (t1 - t0) * 1e3 / steps
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_512_multiphase.json")
+
+
+def measured_traffic(stage: str, n: int):
+    """HBM bytes per launch of a stage's kernel from the committed rocprofv3
+    --pmc summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter
+    reads for a plain copy of known size), or None if that file does not
+    cover this stage at this size."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("n") != n:
+        return None
+    k = t.get("stages", {}).get(stage)
+    return None if k is None else float(k["read_bytes"] + k["write_bytes"])
+
+
 def cpu_baseline(args, fields, tracers):
     """Time the CPU oracle on one step of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -256,7 +275,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(dom_gbps / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(dom["stage"], args.n),
             "avg_ms": round(dom_avg_ms, 4),
             "bytes_per_launch": dom["bytes_per_call"],
             "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
