@@ -154,6 +154,33 @@ contains
         'mhit_upload_tracer_state')
   end subroutine
 
+  ! Drop centres in the format of post/unionfind_periodic/main.f90
+  ! (write_centers): center_timestep_%08d.dat with "# ID x y z u v w volume"
+  ! and one line per drop, plus a line "step count" appended to
+  ! drop_count.dat.  Connectivity 6 (faces), threshold 0.5.
+  subroutine write_drops(ctx, t)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: t
+    integer(c_int64_t), parameter :: maxd = 1000000_c_int64_t
+    type(mhit_drop), allocatable :: d(:)
+    integer(c_int64_t) :: nd, m
+    character(len=256) :: name
+    integer :: u
+    allocate(d(maxd))
+    call check(mhit_drop_stats(ctx, 0.5d0, 6_c_int, maxd, d, nd), 'mhit_drop_stats')
+    write(name,'(a,i8.8,a)') './output/center_timestep_', t, '.dat'
+    open(newunit=u, file=trim(name), status='replace', action='write')
+    write(u,*) '# ID x y z u v w volume'
+    do m = 1, min(nd, maxd)
+      write(u,'(I8,1X,7E20.10)') m, d(m)%x, d(m)%y, d(m)%z, d(m)%u, d(m)%v, d(m)%w, d(m)%volume
+    end do
+    close(u)
+    open(newunit=u, file='./output/drop_count.dat', status='unknown', position='append', &
+         action='write')
+    write(u,'(I12,2X,I8)') t, nd
+    close(u)
+  end subroutine
+
   ! AB2 coefficients and gamma of the next step: raw FP64 alpha, beta, gumax.
   subroutine write_ab2(t, alpha, beta, gumax)
     integer, intent(in) :: t

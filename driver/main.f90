@@ -7,10 +7,12 @@
 !
 ! What the reference fixes at compile time is given on the command line:
 !   mhit36 [--nx N] [--phi 0|1] [--part 0|1] [--fp64-poisson] [--k2-modified]
-!          [--trilinear] [--seed S] [--exact-restart] [--quiet]
+!          [--trilinear] [--seed S] [--exact-restart] [--drops] [--quiet]
 ! --exact-restart also dumps the AB2 history (rhs*_o fields, tracer
 ! velocities, alpha/beta/gumax) and, on restart, reads it back so that the
 ! run continues bit for bit instead of restarting with an Euler step.
+! --drops writes the drop statistics of post/unionfind_periodic at every
+! dump (output/center_timestep_%08d.dat, output/drop_count.dat).
 ! (defaults: nx = 256 as multi/module.f90:2, phiflag = partflag = 0 as
 ! multi/main.f90:55-57, FP32 Poisson as the mixedp branch).
 program mhit36
@@ -23,7 +25,7 @@ program mhit36
   type(mhit_diag) :: d
   type(c_ptr) :: ctx
   integer :: n, t, tstart, rc, verbose, f
-  logical :: exact
+  logical :: exact, drops
   real(c_double) :: alpha, beta, gumax
   integer(8) :: seed, count0, count1, rate, countt
   real(c_double), allocatable :: buf(:,:,:)
@@ -34,6 +36,7 @@ program mhit36
   seed = 1_8
   verbose = 1
   exact = .false.
+  drops = .false.
   call parse_args()
   n = p%nx
 
@@ -175,6 +178,7 @@ contains
     end do
     if (p%phiflag == 1) call write_field(ctx, FIELD_PHI, tt, n, buf)
     if (p%partflag == 1) call write_tracers(ctx, tt, exact)
+    if (drops .and. p%phiflag == 1) call write_drops(ctx, tt)
     if (.not. exact) return
     do ff = FIELD_RHSU_O, FIELD_RHSW_O
       call write_field(ctx, ff, tt, n, buf)
@@ -207,6 +211,8 @@ contains
         p%interp_mode = MHIT_INTERP_TRILINEAR
       case ('--exact-restart')
         exact = .true.
+      case ('--drops')
+        drops = .true.
       case ('--quiet')
         verbose = 0
       case default

@@ -28,6 +28,12 @@ module mhit_c
     integer(c_int32_t) :: nx, phiflag, partflag, poisson_precision, k2_mode, interp_mode, cfl_abort
   end type mhit_params
 
+  ! one connected drop of phi >= threshold (mhit_drop_stats)
+  type, bind(c) :: mhit_drop
+    integer(c_int64_t) :: cells, first_cell
+    real(c_double)     :: x, y, z, u, v, w, volume, d_eq
+  end type mhit_drop
+
   type, bind(c) :: mhit_diag
     integer(c_int64_t) :: step
     real(c_double)     :: alpha, beta, gumax, cou, umean, vmean, wmean
@@ -66,6 +72,16 @@ module mhit_c
       type(c_ptr), value :: ctx
       integer(c_int64_t), value :: n
       real(c_double), intent(in) :: x(*), y(*), z(*)
+    end function
+    integer(c_int) function mhit_drop_stats(ctx, threshold, connectivity, max_drops, drops, ndrops) &
+        bind(c, name='mhit_drop_stats')
+      import :: c_int, c_ptr, c_double, c_int64_t, mhit_drop
+      type(c_ptr), value :: ctx
+      real(c_double), value :: threshold
+      integer(c_int), value :: connectivity
+      integer(c_int64_t), value :: max_drops
+      type(mhit_drop), intent(out) :: drops(*)
+      integer(c_int64_t), intent(out) :: ndrops
     end function
     integer(c_int) function mhit_set_ab2_state(ctx, alpha, beta, gumax) &
         bind(c, name='mhit_set_ab2_state')

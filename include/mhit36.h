@@ -183,6 +183,34 @@ int mhit_set_tuning(mhit_ctx* ctx, const mhit_tuning* t);
  * continues exactly as if the call had not been made.                     */
 int mhit_autotune(mhit_ctx* ctx, int steps, mhit_tuning* out);
 
+/* Drop statistics from the phase field (post-processing of the reference,
+ * run on the live state): cells with phi >= threshold, connected through
+ * faces (connectivity 6) or faces, edges and corners (26), across the
+ * periodic boundaries (post/unionfind_periodic/main.f90).  Drops are
+ * numbered by their smallest cell index (z, y, x order), as the reference's
+ * minimum labels; *ndrops returns the total, at most max_drops are written.
+ * Centre: circular mean of the cell centres (i - 1/2) dx on each axis;
+ * velocity: mean over the drop's cells; volume = cells dx^3;
+ * d_eq = (6 volume / pi)^(1/3) (post/DropCountSize/calculate_deq.f90).     */
+typedef struct mhit_drop {
+    int64_t cells;        /* voxel count                                   */
+    int64_t first_cell;   /* smallest linear cell index (z n + y) n + x    */
+    double x, y, z;       /* centre in [0, 2 pi)                           */
+    double u, v, w;       /* mean velocity                                 */
+    double volume, d_eq;
+} mhit_drop;
+int mhit_drop_stats(mhit_ctx* ctx, double threshold, int connectivity,
+                    int64_t max_drops, mhit_drop* out, int64_t* ndrops);
+
+/* Kinetic-energy shell spectrum of the current velocity (the reference
+ * computes spectra offline, python_utilities/plot_fields.py:116-151):
+ * E[k] = 1/2 sum over modes with |m| in [k - 1/2, k + 1/2) of
+ * (|u^|^2 + |v^|^2 + |w^|^2) / n^6, so that sum_k E[k] is mhit_diag's ke.
+ * Uses the context's FFT precision.  *nshells returns the number of shells
+ * (floor(sqrt(3) n/2 + 1/2) + 1); E gets min(nbins, *nshells) values,
+ * zero-filled up to nbins.                                                */
+int mhit_energy_spectrum(mhit_ctx* ctx, int32_t nbins, double* E, int32_t* nshells);
+
 /* Block until all work queued on the context's stream is done.            */
 int mhit_sync(mhit_ctx* ctx);
 

@@ -105,6 +105,19 @@ class Tuning(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Drop(ctypes.Structure):
+    """mhit_drop: one connected drop of the phase field."""
+
+    _fields_ = [
+        ("cells", ctypes.c_int64),
+        ("first_cell", ctypes.c_int64),
+        ("x", ctypes.c_double), ("y", ctypes.c_double), ("z", ctypes.c_double),
+        ("u", ctypes.c_double), ("v", ctypes.c_double), ("w", ctypes.c_double),
+        ("volume", ctypes.c_double),
+        ("d_eq", ctypes.c_double),
+    ]
+
+
 PROF_MAX = 24
 
 
@@ -173,6 +186,9 @@ EXPORTS = [
     ("mhit_get_tuning", ctypes.c_int, [_vp, _P(Tuning)]),
     ("mhit_set_tuning", ctypes.c_int, [_vp, _P(Tuning)]),
     ("mhit_autotune", ctypes.c_int, [_vp, ctypes.c_int, _P(Tuning)]),
+    ("mhit_energy_spectrum", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _P(ctypes.c_int32)]),
+    ("mhit_drop_stats", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int64, _P(Drop),
+                                       _P(ctypes.c_int64)]),
     ("mhit_last_error", ctypes.c_char_p, []),
     ("mhit_init_taylor_green", ctypes.c_int, [_vp]),
     ("mhit_init_drop", ctypes.c_int, [_vp]),

@@ -189,6 +189,28 @@ class Solver:
         _check(self.lib.mhit_autotune(self._h, int(steps), ctypes.byref(t)), "mhit_autotune")
         return t.as_dict()
 
+    def energy_spectrum(self) -> np.ndarray:
+        """E(k), k = 0, 1, ...: shell spectrum of the kinetic energy."""
+        nb = int(np.floor(np.sqrt(3.0) * (self.n // 2) + 0.5)) + 1
+        E = np.zeros(nb)
+        ns = ctypes.c_int32()
+        _check(self.lib.mhit_energy_spectrum(self._h, nb, _ptr(E), ctypes.byref(ns)),
+               "mhit_energy_spectrum")
+        return E[:ns.value]
+
+    def drop_stats(self, threshold: float = 0.5, connectivity: int = 6, max_drops: int = 1 << 16):
+        """Connected drops of phi >= threshold (see mhit_drop_stats); a dict of
+        numpy columns, one entry per drop, ordered by first cell."""
+        buf = (_abi.Drop * max(1, max_drops))()
+        nd = ctypes.c_int64()
+        _check(self.lib.mhit_drop_stats(self._h, float(threshold), int(connectivity), int(max_drops),
+                                        buf, ctypes.byref(nd)), "mhit_drop_stats")
+        k = min(nd.value, max_drops)
+        names = [f for f, _ in _abi.Drop._fields_]
+        out = {f: np.array([getattr(buf[i], f) for i in range(k)]) for f in names}
+        out["count"] = nd.value
+        return out
+
     def profile_enable(self, on: bool = True):
         _check(self.lib.mhit_profile_enable(self._h, int(on)), "mhit_profile_enable")
 

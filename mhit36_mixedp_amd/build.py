@@ -20,7 +20,8 @@ OUT = os.path.join(HERE, "libmhit36.so")
 ARCH = os.environ.get("MHIT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "poisson.hip", "stencil.hip", "march.hip", "tracers.hip", "init.hip",
-           "ensemble.hip", "tune.hip"]
+           "ensemble.hip", "tune.hip", "drops.hip",
+           "spectra.hip"]
 
 CXXFLAGS = [
     f"--offload-arch={ARCH}",

@@ -912,3 +912,66 @@ int oracle_diag(oracle_ctx* c, mhit_diag* d) {
     d->nan_count = nans;
     return MHIT_OK;
 }
+
+/* ------------------------------------------------------------------ */
+/* drop statistics, post/unionfind_periodic/main.f90:97-441 and
+ * post/DropCountSize/calculate_deq.f90.  The reference labels by iterated
+ * minimum-label propagation; a flood fill from each unvisited masked cell
+ * in scan order gives the same components with the same numbering (a
+ * component's first cell in scan order is its minimum label).            */
+int oracle_drop_stats(int n, const double* phi, const double* u, const double* v,
+                      const double* w, double thr, int conn, int64_t maxd, double* out,
+                      int64_t* nd) {
+    if (conn != 6 && conn != 26) return 1;
+    const size_t N3 = (size_t)n * n * n;
+    const double pi = 4.0 * atan(1.0), lx = 2.0 * pi, dx = lx / (double)n;
+    double* x = (double*)malloc(sizeof(double) * n);   /* cell centres, main.f90:295-299 */
+    unsigned char* seen = (unsigned char*)calloc(N3, 1);
+    int64_t* queue = (int64_t*)malloc(sizeof(int64_t) * N3);
+    if (!x || !seen || !queue) { free(x); free(seen); free(queue); return 2; }
+    x[0] = dx / 2.0;
+    for (int i = 1; i < n; i++) x[i] = x[i - 1] + dx;
+    int64_t count = 0;
+    for (size_t q0 = 0; q0 < N3; q0++) {
+        if (seen[q0] || !(phi[q0] >= thr)) continue;
+        double cells = 0, sx = 0, cx = 0, sy = 0, cy = 0, sz = 0, cz = 0, su = 0, sv = 0, sw = 0;
+        size_t head = 0, tail = 0;
+        queue[tail++] = (int64_t)q0;
+        seen[q0] = 1;
+        while (head < tail) {
+            const size_t q = (size_t)queue[head++];
+            const int i = (int)(q % n), j = (int)((q / n) % n), k = (int)(q / ((size_t)n * n));
+            cells += 1.0;
+            sx += sin(2.0 * pi * x[i] / lx); cx += cos(2.0 * pi * x[i] / lx);
+            sy += sin(2.0 * pi * x[j] / lx); cy += cos(2.0 * pi * x[j] / lx);
+            sz += sin(2.0 * pi * x[k] / lx); cz += cos(2.0 * pi * x[k] / lx);
+            if (u) { su += u[q]; sv += v[q]; sw += w[q]; }
+            for (int dk = -1; dk <= 1; dk++)
+                for (int dj = -1; dj <= 1; dj++)
+                    for (int di = -1; di <= 1; di++) {
+                        const int m = abs(di) + abs(dj) + abs(dk);
+                        if (m == 0 || (conn == 6 && m != 1)) continue;
+                        const size_t r = ((size_t)wrapi(k + dk, n) * n + wrapi(j + dj, n)) * n +
+                                         wrapi(i + di, n);
+                        if (!seen[r] && phi[r] >= thr) { seen[r] = 1; queue[tail++] = (int64_t)r; }
+                    }
+        }
+        if (count < maxd) {
+            double* o = out + (size_t)count * 10;
+            double xc = lx * atan2(sx, cx) / (2.0 * pi), yc = lx * atan2(sy, cy) / (2.0 * pi),
+                   zc = lx * atan2(sz, cz) / (2.0 * pi);
+            if (xc < 0.0) xc += lx;
+            if (yc < 0.0) yc += lx;
+            if (zc < 0.0) zc += lx;
+            o[0] = cells; o[1] = (double)q0;
+            o[2] = xc; o[3] = yc; o[4] = zc;
+            o[5] = su / cells; o[6] = sv / cells; o[7] = sw / cells;
+            o[8] = cells * dx * dx * dx;
+            o[9] = cbrt(6.0 * o[8] / pi);
+        }
+        count++;
+    }
+    free(x); free(seen); free(queue);
+    *nd = count;
+    return 0;
+}

@@ -52,6 +52,16 @@ int    oracle_divergence(oracle_ctx* c, double* rhsp);
 int    oracle_interp(oracle_ctx* c, double* up, double* vp, double* wp);
 int    oracle_poisson(oracle_ctx* c, const double* rhs, double* p);
 
+/* Drop statistics of post/unionfind_periodic/main.f90 (mask phi >= thr,
+ * 6- or 26-point connectivity with periodic wrap) plus d_eq of
+ * post/DropCountSize/calculate_deq.f90.  Drops are numbered in the order of
+ * their first cell (z, y, x scan); each writes 10 values to out: cells,
+ * first cell, centre x y z, mean u v w, volume, d_eq.  u, v, w may be NULL
+ * (velocities then 0).  Returns the number of drops in *nd.               */
+int    oracle_drop_stats(int n, const double* phi, const double* u,
+                         const double* v, const double* w, double thr,
+                         int conn, int64_t maxd, double* out, int64_t* nd);
+
 /* FFT building blocks: complex interleaved (re,im), unnormalised,
  * sign = -1 forward (e^{-i}), +1 inverse.                                  */
 void   oracle_fft1d(int n, const double* in, double* out, int sign);

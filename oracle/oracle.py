@@ -48,6 +48,8 @@ _SIGS = [
     ("oracle_poisson", ctypes.c_int, [_vp, _dp, _dp]),
     ("oracle_fft1d", None, [ctypes.c_int, _dp, _dp, ctypes.c_int]),
     ("oracle_fft3d", None, [ctypes.c_int, _dp, _dp, ctypes.c_int]),
+    ("oracle_drop_stats", ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int64, _dp, _P(ctypes.c_int64)]),
     ("oracle_fft1d_f32", None, [ctypes.c_int, _fp, _fp, ctypes.c_int]),
 ]
 
@@ -94,6 +96,26 @@ def fft1d_f32(x: np.ndarray, sign: int = -1) -> np.ndarray:
     out = np.empty_like(x)
     lib().oracle_fft1d_f32(x.size, x.ctypes.data_as(_fp), out.ctypes.data_as(_fp), sign)
     return out
+
+
+def drop_stats(phi: np.ndarray, u=None, v=None, w=None, threshold: float = 0.5,
+               connectivity: int = 6, max_drops: int = 1 << 16) -> dict:
+    """Drop statistics of post/unionfind_periodic (CPU restatement)."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    n = phi.shape[0]
+    out = np.zeros((max(1, max_drops), 10))
+    nd = ctypes.c_int64()
+    vel = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w)]
+    ptrs = [None if a is None else _dptr(a) for a in vel]
+    rc = lib().oracle_drop_stats(n, _dptr(phi), *ptrs, float(threshold), int(connectivity),
+                                 int(max_drops), _dptr(out), ctypes.byref(nd))
+    if rc:
+        raise RuntimeError(f"oracle_drop_stats failed: {rc}")
+    k = min(nd.value, max_drops)
+    cols = ["cells", "first_cell", "x", "y", "z", "u", "v", "w", "volume", "d_eq"]
+    res = {c: out[:k, i].copy() for i, c in enumerate(cols)}
+    res["count"] = nd.value
+    return res
 
 
 def fft3d(x: np.ndarray, sign: int = -1) -> np.ndarray:

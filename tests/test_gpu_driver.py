@@ -68,3 +68,23 @@ def test_driver_fresh_start_and_restart(tmp_path):
         s.step(2)
         assert np.array_equal(s.download_field(1), read(tmp_path, "u_00000005.dat", n))
         assert np.array_equal(s.download_field(5), read(tmp_path, "phi_00000005.dat", n))
+
+
+def test_driver_drop_centres(tmp_path, oracle_mod):
+    """--drops writes post/unionfind_periodic's center_timestep file at each
+    dump; its rows equal the oracle's statistics of the dumped fields."""
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built (no amdflang)")
+    n = 32
+    os.makedirs(tmp_path / "output")
+    write_inp(tmp_path, 0, 0, 4, 2, npart=0)
+    run(str(tmp_path), "--nx", str(n), "--phi", "1", "--drops", "--quiet")
+    phi = read(tmp_path, "phi_00000004.dat", n)
+    u, v, w = (read(tmp_path, f"{f}_00000004.dat", n) for f in ("u", "v", "w"))
+    o = oracle_mod.drop_stats(phi, u, v, w)
+    rows = np.loadtxt(tmp_path / "output" / "center_timestep_00000004.dat", comments="#", ndmin=2)
+    assert rows.shape[0] == o["count"] >= 1
+    for col, k in enumerate(("x", "y", "z", "u", "v", "w", "volume"), start=1):
+        assert np.allclose(rows[:, col], o[k], rtol=1e-8, atol=1e-10), k
+    counts = np.loadtxt(tmp_path / "output" / "drop_count.dat", ndmin=2)
+    assert counts[:, 0].tolist() == [0, 2, 4] and counts[-1, 1] == o["count"]
