@@ -32,10 +32,13 @@ def log(*a):
 
 
 def step_bytes_fused(n: int, ntr: int, phase: bool, fp32: bool) -> float:
-    """Compulsory HBM bytes per step with every array read and written once
-    per pass: 208/248 B per cell (NS, FP32/FP64 Poisson), 272/312 with the
-    phase field, plus 144 B per tracer."""
-    per_cell = (272.0 if fp32 else 312.0) if phase else (208.0 if fp32 else 248.0)
+    """HBM bytes per step of this pass structure with every array read and
+    written once per pass, plus 144 B per tracer.  The reference's pass
+    structure needs 208/248 B per cell (single phase, FP32/FP64 Poisson) and
+    272/312 with the phase field; here the NS pass also writes div u* (4/8 B)
+    and the x R2C pass reads it instead of u*, v*, w* (24 B): 192/240 and
+    256/304 B per cell."""
+    per_cell = (256.0 if fp32 else 304.0) if phase else (192.0 if fp32 else 240.0)
     return per_cell * n ** 3 + 144.0 * ntr
 
 
@@ -97,6 +100,43 @@ def cpu_baseline(args, fields, tracers):
     if rc != 0:
         raise RuntimeError(f"oracle step failed: {rc}")
     return (t1 - t0) * 1e3 / args.cpu_steps, threads
+
+
+def small_box_comparison(args):
+    """The reference's CPU case: 64^3 single-phase NS, FP64 throughout,
+    nu = 0.006, dt = 5e-4, random-phase |k| <= 8 field.  GPU ms/step over 100
+    steps; CPU oracle median ms/step over 10 steps after 5 warm-up steps, on
+    1 thread and on all threads used for the baseline."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle
+    from mhit36_mixedp_amd import Solver
+    kw = dict(nx=64, poisson_precision=0, phiflag=0, partflag=0, dt=5e-4, mu=0.006)
+    with Solver(**kw) as s:
+        s.init_random_velocity(args.seed, kmax=8.0, urms=1.0)
+        vel = s.velocity()
+        s.step(5)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(100)
+        gpu_ms = (time.perf_counter() - t0) * 1e3 / 100
+    res = {"workload": "64^3 single-phase NS, FP64 Poisson, nu=0.006, dt=5e-4",
+           "gpu_ms_per_step": round(gpu_ms, 4)}
+    threads_all = min(16, os.cpu_count() or 1)
+    for th in (1, threads_all):
+        oracle.set_threads(th)
+        o = oracle.Oracle(**kw)
+        for f, a in zip((1, 2, 3), vel):
+            o.set_field(f, a)
+        o.step(5)
+        ts = []
+        for _ in range(10):
+            t0 = time.perf_counter()
+            o.step(1)
+            ts.append((time.perf_counter() - t0) * 1e3)
+        o.close()
+        res[f"cpu_ms_per_step_{th}_threads"] = round(float(np.median(ts)), 3)
+    return res
 
 
 ENSEMBLE_SIZES = (64, 96, 128)
@@ -243,6 +283,11 @@ def main():
                          f"({args.ndrops} drops, {args.ntracers} tracers, FP32 Poisson) on the CPU "
                          "FP64 restatement (oracle/), OpenMP"}
 
+    small = None
+    if not args.no_cpu_baseline:
+        small = small_box_comparison(args)
+        log(f"[bench] 64^3 NS: {small}")
+
     out = {
         "metric": "ms/timestep & HBM GB/s at 512^3 multiphase, FP32 vs FP64 Poisson, 1 MI355X",
         "value": round(ms32, 4),
@@ -282,6 +327,7 @@ def main():
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
         "ensemble": ens,
+        "small_box": small,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

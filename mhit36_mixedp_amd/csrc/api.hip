@@ -129,9 +129,10 @@ double stage_bytes(const mhit_ctx* c, int s) {
         case ST_ACDI_UPDATE: return 72.0 * N3;               // r phi,rphi,rphi_o,T,n x3; w phi,rphi_o
         case ST_ACDI: return 56.0 * N3;                      // r phi,u,v,w,rphi_o; w phi',rphi_o
         case ST_CSF: return 32.0 * N3;                       // r phi; w f x3
-        case ST_NS:   // r u,v,w,rhs_o[,phi or f x3]; w u*,rhs_o
-            return (c->P.phiflag ? (c->march ? 104.0 : 120.0) : 96.0) * N3;
-        case ST_FFT_X_FWD: return 24.0 * N3 + spec;          // r u*,v*,w*; w spectrum
+        case ST_NS:   // r u,v,w,rhs_o[,phi or f x3]; w u*,rhs_o[, div u* (fused pass)]
+            return ((c->P.phiflag ? (c->march ? 104.0 : 120.0) : 96.0) + (c->march ? rs : 0.0)) * N3;
+        case ST_FFT_X_FWD:   // r div u* (fused NS) or u*,v*,w*; w spectrum
+            return (c->march ? rs : 24.0) * N3 + spec;
         case ST_FFT_Y_FWD: case ST_FFT_Z: case ST_FFT_Y_INV: return 2.0 * spec;
         case ST_FFT_X_INV: return spec + rs * N3;            // r spectrum; w p
         case ST_CORRECT: return (48.0 + rs) * N3;            // r u*, p; w u
@@ -166,9 +167,15 @@ void clear_graphs(mhit_ctx* c) {
 // and phi_next; a tracer re-ordering swaps the tracer columns).  All
 // per-step scalars live in device memory, so replays need no host update.
 static int graph_step(mhit_ctx* c) {
+    double* cur[mhit_ctx::kPingPong];
+    pingpong_current(c, cur);
     mhit_ctx::StepGraph* slot = nullptr;
-    for (auto& gr : c->graphs)
-        if (gr.exec && gr.phi_in == c->phi && gr.tag == c->ptag) { slot = &gr; break; }
+    for (auto& gr : c->graphs) {
+        if (!gr.exec || gr.tag != c->ptag) continue;
+        bool same = true;
+        for (int p = 0; p < mhit_ctx::kPingPong; p++) same = same && gr.in[p] == cur[p];
+        if (same) { slot = &gr; break; }
+    }
     if (!slot) {
         for (auto& gr : c->graphs)
             if (!gr.exec) { slot = &gr; break; }
@@ -176,27 +183,24 @@ static int graph_step(mhit_ctx* c) {
             clear_graphs(c);
             slot = &c->graphs[0];
         }
-        double* phi_in = c->phi;
         double* tag = c->ptag;
         hipGraph_t graph = nullptr;
         MHIT_HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        int rc = enqueue_step_kernels(c);    // swaps phi/phi_next on the host
+        int rc = enqueue_step_kernels(c);    // performs the role swaps on the host
         hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        // the capture advanced the host-side buffer roles; undo it so the
+        // launch below applies the step once
+        for (int p = 0; p < mhit_ctx::kPingPong; p++) slot->in[p] = cur[p];
+        pingpong_undo(c, slot->in, slot->out);
         if (rc) { if (graph) (void)hipGraphDestroy(graph); return rc; }
         MHIT_HIP_TRY(e);
         e = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         MHIT_HIP_TRY(e);
-        slot->phi_in = phi_in;
-        slot->phi_out = c->phi;
         slot->tag = tag;
-        // the capture advanced the host-side buffer roles; undo it so the
-        // launch below applies the step once
-        c->phi = phi_in;
-        if (c->phi_next == phi_in) c->phi_next = slot->phi_out;
     }
     MHIT_HIP_TRY(hipGraphLaunch(slot->exec, c->stream));
-    if (slot->phi_out != slot->phi_in) std::swap(c->phi, c->phi_next);
+    pingpong_apply(c, slot->in, slot->out);
     return MHIT_OK;
 }
 
@@ -353,7 +357,8 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
         return fail(MHIT_ERR_HIP);
     }
     const size_t N3 = c->N3;
-    double** fields[] = {&c->u, &c->v, &c->w, &c->us, &c->vs, &c->ws, &c->rhsu_o, &c->rhsv_o, &c->rhsw_o};
+    double** fields[] = {&c->u, &c->v, &c->w, &c->us, &c->vs, &c->ws, &c->rhsu_o, &c->rhsv_o, &c->rhsw_o,
+                         &c->rhsu_n, &c->rhsv_n, &c->rhsw_n};
     for (double** f : fields)
         if ((rc = dmalloc(f, N3))) return fail(rc);
     if (P->phiflag) {
@@ -438,6 +443,7 @@ int mhit_destroy(mhit_ctx* c) {
     if (!c) return MHIT_OK;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void* ptrs[] = {c->u, c->v, c->w, c->us, c->vs, c->ws, c->rhsu_o, c->rhsv_o, c->rhsw_o,
+                    c->rhsu_n, c->rhsv_n, c->rhsw_n,
                     c->phi, c->phi_next, c->rhsphi, c->rhsphi_o, c->psi, c->tanh_psi, c->normx, c->normy,
                     c->normz, c->fxst, c->fyst, c->fzst, c->pres, c->spec, c->nyq, c->tw, c->kk,
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,

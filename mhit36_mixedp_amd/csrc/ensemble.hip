@@ -7,7 +7,7 @@
 // CUs alone) at the cost of a single launch.  The tracer re-ordering
 // (hipCUB radix sort, every sort_every steps) runs eagerly on the ensemble
 // stream before the replay.  Graphs are keyed by every box's current
-// buffers (phi ping-pong parity, tracer columns).  Boxes share nothing; a
+// buffers (the ping-pong parities, tracer columns).  Boxes share nothing; a
 // box stopped by the CFL check stays frozen (its kernels return at once)
 // while the others continue.
 #include "internal.hpp"
@@ -26,8 +26,13 @@ using namespace mhit;
 
 struct EnsembleGraph {
     hipGraphExec_t exec = nullptr;
-    std::vector<double*> phi_in, phi_out, tag;
-    std::vector<unsigned long long> gen;   // each box's graph generation at capture
+    struct Box {
+        double* in[mhit_ctx::kPingPong];    // roles of the ping-pong buffers before the step
+        double* out[mhit_ctx::kPingPong];   // ... and after it
+        double* tag;                        // first tracer column
+        unsigned long long gen;             // the box's graph generation at capture
+    };
+    std::vector<Box> box;
 };
 
 struct mhit_ensemble {
@@ -48,23 +53,25 @@ static void clear_ensemble_graphs(mhit_ensemble* e) {
 
 static bool graph_matches(const EnsembleGraph& gr, const mhit_ensemble* e) {
     if (!gr.exec) return false;
-    for (size_t b = 0; b < e->boxes.size(); b++)
-        if (gr.phi_in[b] != e->boxes[b]->phi || gr.tag[b] != e->boxes[b]->ptag ||
-            gr.gen[b] != e->boxes[b]->graph_gen)
-            return false;
+    for (size_t b = 0; b < e->boxes.size(); b++) {
+        mhit_ctx* c = e->boxes[b];
+        double* cur[mhit_ctx::kPingPong];
+        pingpong_current(c, cur);
+        const EnsembleGraph::Box& g = gr.box[b];
+        if (g.tag != c->ptag || g.gen != c->graph_gen) return false;
+        for (int p = 0; p < mhit_ctx::kPingPong; p++)
+            if (g.in[p] != cur[p]) return false;
+    }
     return true;
 }
 
 static int capture_ensemble_step(mhit_ensemble* e, EnsembleGraph* slot) {
     const size_t nb = e->boxes.size();
-    slot->phi_in.resize(nb);
-    slot->phi_out.resize(nb);
-    slot->tag.resize(nb);
-    slot->gen.resize(nb);
+    slot->box.resize(nb);
     for (size_t b = 0; b < nb; b++) {
-        slot->phi_in[b] = e->boxes[b]->phi;
-        slot->tag[b] = e->boxes[b]->ptag;
-        slot->gen[b] = e->boxes[b]->graph_gen;
+        pingpong_current(e->boxes[b], slot->box[b].in);
+        slot->box[b].tag = e->boxes[b]->ptag;
+        slot->box[b].gen = e->boxes[b]->graph_gen;
     }
     hipGraph_t graph = nullptr;
     MHIT_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
@@ -73,11 +80,14 @@ static int capture_ensemble_step(mhit_ensemble* e, EnsembleGraph* slot) {
     for (size_t b = 0; b < nb && err == hipSuccess && rc == MHIT_OK; b++) {
         mhit_ctx* c = e->boxes[b];
         err = hipStreamWaitEvent(c->stream, e->fork, 0);   // the box stream joins the capture
-        if (err == hipSuccess) rc = step_kernels(c);        // swaps phi/phi_next on the host
+        if (err == hipSuccess) rc = step_kernels(c);        // performs the role swaps on the host
         if (err == hipSuccess && rc == MHIT_OK) err = hipEventRecord(e->join[b], c->stream);
         if (err == hipSuccess && rc == MHIT_OK) err = hipStreamWaitEvent(e->stream, e->join[b], 0);
     }
     hipError_t end = hipStreamEndCapture(e->stream, &graph);
+    // the capture advanced each box's host-side buffer roles; undo it so the
+    // launch applies the step once
+    for (size_t b = 0; b < nb; b++) pingpong_undo(e->boxes[b], slot->box[b].in, slot->box[b].out);
     if (rc != MHIT_OK || err != hipSuccess || end != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         if (rc != MHIT_OK) return rc;
@@ -86,14 +96,6 @@ static int capture_ensemble_step(mhit_ensemble* e, EnsembleGraph* slot) {
     err = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     MHIT_HIP_TRY(err);
-    // the capture advanced each box's host-side buffer roles; undo it so the
-    // launch applies the step once
-    for (size_t b = 0; b < nb; b++) {
-        mhit_ctx* c = e->boxes[b];
-        slot->phi_out[b] = c->phi;
-        c->phi = slot->phi_in[b];
-        if (c->phi_next == slot->phi_in[b]) c->phi_next = slot->phi_out[b];
-    }
     return MHIT_OK;
 }
 
@@ -122,7 +124,7 @@ static int ensemble_graph_step(mhit_ensemble* e) {
     }
     MHIT_HIP_TRY(hipGraphLaunch(slot->exec, e->stream));
     for (size_t b = 0; b < e->boxes.size(); b++)
-        if (slot->phi_out[b] != slot->phi_in[b]) std::swap(e->boxes[b]->phi, e->boxes[b]->phi_next);
+        pingpong_apply(e->boxes[b], slot->box[b].in, slot->box[b].out);
     return MHIT_OK;
 }
 

@@ -4,6 +4,8 @@
 
 #include "common.hpp"
 
+#include <utility>
+
 struct mhit_ctx {
     mhit_params P;
     int n = 0;
@@ -18,6 +20,9 @@ struct mhit_ctx {
     // provisional velocity u* (main.f90:783-796)
     double *us = nullptr, *vs = nullptr, *ws = nullptr;
     double *rhsu_o = nullptr, *rhsv_o = nullptr, *rhsw_o = nullptr;
+    // the other half of the AB2 history: the fused NS pass reads its
+    // neighbours' previous rhs, so it writes the new one here and swaps
+    double *rhsu_n = nullptr, *rhsv_n = nullptr, *rhsw_n = nullptr;
     // phase field (module.f90:66-71)
     double *phi = nullptr, *rhsphi = nullptr, *rhsphi_o = nullptr;
     double *phi_next = nullptr;   // ping-pong target of the fused ACDI pass
@@ -58,10 +63,12 @@ struct mhit_ctx {
     mhit::DevScalars S_host{};
     // captured timesteps, keyed by the buffers they use (phi ping-pong
     // parity, tracer columns after a re-ordering)
+    // buffer pairs whose roles alternate from step to step
+    static constexpr int kPingPong = 4;   // phi, rhsu_o, rhsv_o, rhsw_o
     struct StepGraph {
         hipGraphExec_t exec = nullptr;
-        double* phi_in = nullptr;      // phi before the step
-        double* phi_out = nullptr;     // phi after the step
+        double* in[kPingPong] = {};    // current buffer of each pair before the step
+        double* out[kPingPong] = {};   // ... and after it
         double* tag = nullptr;         // first tracer column
     };
     bool use_graph = true;
@@ -87,6 +94,44 @@ enum Stage {
 };
 const char* stage_name(int s);
 double stage_bytes(const mhit_ctx* c, int s);
+
+// The buffer pairs whose roles a timestep swaps on the host (the fused
+// ACDI pass's phi ping-pong, the fused NS pass's AB2-history ping-pong).
+// A captured step replays with the roles it was captured with, so graphs
+// are keyed by the current buffer of each pair.
+inline void pingpong_pairs(mhit_ctx* c, double** a[mhit_ctx::kPingPong],
+                           double** b[mhit_ctx::kPingPong]) {
+    a[0] = &c->phi;    b[0] = &c->phi_next;
+    a[1] = &c->rhsu_o; b[1] = &c->rhsu_n;
+    a[2] = &c->rhsv_o; b[2] = &c->rhsv_n;
+    a[3] = &c->rhsw_o; b[3] = &c->rhsw_n;
+}
+// After a capture (which performed the swaps on the host): record the
+// roles after the step in out[], and put the roles back to in[].
+inline void pingpong_undo(mhit_ctx* c, double* const in[], double* out[]) {
+    double** a[mhit_ctx::kPingPong];
+    double** b[mhit_ctx::kPingPong];
+    pingpong_pairs(c, a, b);
+    for (int p = 0; p < mhit_ctx::kPingPong; p++) {
+        out[p] = *a[p];
+        *a[p] = in[p];
+        if (*b[p] == in[p]) *b[p] = out[p];
+    }
+}
+// After a replay: apply the swaps the captured step performs.
+inline void pingpong_apply(mhit_ctx* c, double* const in[], double* const out[]) {
+    double** a[mhit_ctx::kPingPong];
+    double** b[mhit_ctx::kPingPong];
+    pingpong_pairs(c, a, b);
+    for (int p = 0; p < mhit_ctx::kPingPong; p++)
+        if (out[p] != in[p]) std::swap(*a[p], *b[p]);
+}
+inline void pingpong_current(mhit_ctx* c, double* in[]) {
+    double** a[mhit_ctx::kPingPong];
+    double** b[mhit_ctx::kPingPong];
+    pingpong_pairs(c, a, b);
+    for (int p = 0; p < mhit_ctx::kPingPong; p++) in[p] = *a[p];
+}
 
 // Record the end of stage s on the context stream (profiling only).
 inline void prof_mark(mhit_ctx* c, int s) {

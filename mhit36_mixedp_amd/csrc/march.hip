@@ -18,6 +18,8 @@
 // each intermediate lives changes.
 #include "internal.hpp"
 
+#include <type_traits>
+
 namespace mhit {
 
 __device__ __forceinline__ int wrapn(int a, int n) {
@@ -28,26 +30,46 @@ __device__ __forceinline__ int ring3(int k) { return ((k % 3) + 3) % 3; }
 __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 
 // =====================================================================
-// NS + CSF + AB2
+// NS + CSF + AB2 + divergence of u*
 // =====================================================================
+// Besides u* and rhs_o, the pass writes the divergence of u* that opens
+// the Poisson solve (main.f90:833-848), so that the x R2C pass reads one
+// scalar field instead of u*, v*, w*.  The divergence needs u*(i+1),
+// v*(j+1) and w*(k+1): u* and v* of the plane are exchanged through LDS,
+// with the column x = MX and the row y = MY just past the tile evaluated by
+// two waves (the same expressions as the neighbouring tile's own, so the
+// same bits); each thread carries its partial sum to the next plane, where
+// w*(k+1) is known, so a chunk marches one plane past its end, computing
+// only w* there.  Threads of a partial tile past the domain edge compute
+// the periodic image of their cell (their results feed the divergence of
+// the last cells but are not stored).  The velocity planes carry a
+// two-point halo on the +x/+y sides for these evaluations, and the fused
+// pass reads its neighbours' previous rhs, so the new AB2 history goes to
+// the other buffer of a ping-pong pair.
 constexpr int NS_MX = 64, NS_MY = 16;  // z chunk: Tuning::ns_mz
-constexpr int NS_W1 = NS_MX + 2, NS_H1 = NS_MY + 2;   // halo 1 (velocity)
-constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // halo 2 (phi)
+constexpr int NS_W1 = NS_MX + 3, NS_H1 = NS_MY + 3;   // velocity: x, y in [-1, M + 1]
+constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // phi: halo 2
 
-template <bool PHI>
+template <bool PHI, typename TD>
 __global__ __launch_bounds__(NS_MX * NS_MY)
 void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
                 const double* __restrict__ w, const double* __restrict__ phi,
-                double* __restrict__ ru_o, double* __restrict__ rv_o, double* __restrict__ rw_o,
+                const double* __restrict__ ru_o, const double* __restrict__ rv_o,
+                const double* __restrict__ rw_o, double* __restrict__ ru_n,
+                double* __restrict__ rv_n, double* __restrict__ rw_n,
                 double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
+                TD* __restrict__ div, double cdiv,
                 const double* __restrict__ mysin, const double* __restrict__ mycos,
                 Grid g, const DevScalars* __restrict__ S, int zc, int xcd) {
     constexpr int MX = NS_MX, MY = NS_MY, W1 = NS_W1, H1 = NS_H1, W2 = NS_W2, H2 = NS_H2;
     constexpr int NT = MX * MY;
+    constexpr int FXW = MX + 2;                          // fx at x = -1..MX
     __shared__ double su[3][H1 * W1], sv[3][H1 * W1], sw[3][H1 * W1];
     __shared__ double sp[PHI ? 3 : 1][PHI ? H2 * W2 : 1];
-    __shared__ double sfx[PHI ? MY * (MX + 1) : 1];   // fx at x = -1..MX-1
-    __shared__ double sfy[PHI ? (MY + 1) * MX : 1];   // fy at y = -1..MY-1
+    __shared__ double sfx[PHI ? MY * FXW : 1];
+    __shared__ double sfy[PHI ? (MY + 2) * MX : 1];     // fy at y = -1..MY
+    __shared__ double sus[MY * (MX + 1)];               // u* at x = 0..MX
+    __shared__ double svs[(MY + 1) * MX];               // v* at y = 0..MY
 
     if (S->halted) return;
     const int n = g.n;
@@ -59,6 +81,8 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
+    // global indices of the cell, or of its periodic image past the domain edge
+    const int ic = wrapn(i, n), jc = wrapn(j, n);
     const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
     const double alpha = S->alpha, beta = S->beta;
     const double dxi = g.dxi, ddxi = g.ddxi, mu = g.mu, rhoi = g.rhoi, dt = g.dt;
@@ -123,6 +147,66 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
                eps * (P(k, a + 1, b) + P(k, a - 1, b) + P(k, a, b + 1) + P(k, a, b - 1) +
                       P(k + 1, a, b) + P(k - 1, a, b) - 6.0 * ph) * ddxi;
     };
+    // velocity ring entry at tile coordinates (a, b), a, b from -1
+    auto at = [&](int a, int b) { return (b + 1) * W1 + (a + 1); };
+
+    // x-momentum RHS at tile (a, b) of plane k with global row jg
+    // (main.f90:664-668, 691-693, 711, 760-764), in the reference's order
+    auto rhs_u = [&](int sm, int sc, int sq, int a, int b, int jg, int k) -> double {
+        const int c = at(a, b);
+        const double u0 = su[sc][c], uxp = su[sc][c + 1], uxm = su[sc][c - 1];
+        const double uyp = su[sc][c + W1], uym = su[sc][c - W1], uzp = su[sq][c], uzm = su[sm][c];
+        double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
+        double h12 = (uyp + u0) * (sv[sc][c + W1] + sv[sc][c + W1 - 1]) - (u0 + uym) * (sv[sc][c] + sv[sc][c - 1]);
+        double h13 = (uzp + u0) * (sw[sq][c] + sw[sq][c - 1]) - (u0 + uzm) * (sw[sc][c] + sw[sc][c - 1]);
+        h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
+        double ru = -(h11 + h12 + h13);
+        h11 = mu * (uxp - 2.0 * u0 + uxm) * ddxi;
+        h12 = mu * (uyp - 2.0 * u0 + uym) * ddxi;
+        h13 = mu * (uzp - 2.0 * u0 + uzm) * ddxi;
+        ru = ru + (h11 + h12 + h13) * rhoi;
+        ru = ru + g.f3 * mysin[k + 1] + g.f2 * mycos[jg + 1];
+        if (PHI) ru = ru + 0.5 * (sfx[b * FXW + a] + sfx[b * FXW + a + 1]) * rhoi;
+        return ru;
+    };
+    // y-momentum RHS at tile (a, b) of plane k with global column ig
+    auto rhs_v = [&](int sm, int sc, int sq, int a, int b, int ig, int k) -> double {
+        const int c = at(a, b);
+        const double v0 = sv[sc][c], vxp = sv[sc][c + 1], vxm = sv[sc][c - 1];
+        const double vyp = sv[sc][c + W1], vym = sv[sc][c - W1], vzp = sv[sq][c], vzm = sv[sm][c];
+        const double u0 = su[sc][c], uym = su[sc][c - W1];
+        double h21 = (su[sc][c + 1] + su[sc][c + 1 - W1]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
+        double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
+        double h23 = (sw[sq][c] + sw[sq][c - W1]) * (vzp + v0) - (sw[sc][c] + sw[sc][c - W1]) * (v0 + vzm);
+        h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
+        double rv = -(h21 + h22 + h23);
+        h21 = mu * (vxp - 2.0 * v0 + vxm) * ddxi;
+        h22 = mu * (vyp - 2.0 * v0 + vym) * ddxi;
+        h23 = mu * (vzp - 2.0 * v0 + vzm) * ddxi;
+        rv = rv + (h21 + h22 + h23) * rhoi;
+        rv = rv + g.f1 * mysin[ig] + g.f3 * mycos[k + 1];
+        if (PHI) rv = rv + 0.5 * (sfy[b * MX + a] + sfy[(b + 1) * MX + a]) * rhoi;
+        return rv;
+    };
+    // z-momentum RHS at the thread's own column of plane k
+    auto rhs_w = [&](int sm, int sc, int sq, double fzm, double fzc) -> double {
+        const int c = at(tx, ty);
+        const double w0 = sw[sc][c], wxp = sw[sc][c + 1], wxm = sw[sc][c - 1];
+        const double wyp = sw[sc][c + W1], wym = sw[sc][c - W1], wzp = sw[sq][c], wzm = sw[sm][c];
+        const double u0 = su[sc][c], v0 = sv[sc][c];
+        double h31 = (wxp + w0) * (su[sc][c + 1] + su[sm][c + 1]) - (w0 + wxm) * (u0 + su[sm][c]);
+        double h32 = (sv[sc][c + W1] + sv[sm][c + W1]) * (wyp + w0) - (v0 + sv[sm][c]) * (w0 + wym);
+        double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
+        h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
+        double rw = -(h31 + h32 + h33);
+        h31 = mu * (wxp - 2.0 * w0 + wxm) * ddxi;
+        h32 = mu * (wyp - 2.0 * w0 + wym) * ddxi;
+        h33 = mu * (wzp - 2.0 * w0 + wzm) * ddxi;
+        rw = rw + (h31 + h32 + h33) * rhoi;
+        rw = rw + g.f2 * mysin[jc + 1] + g.f1 * mycos[ic];
+        if (PHI) rw = rw + 0.5 * (fzm + fzc) * rhoi;
+        return rw;
+    };
 
     double fz_prev = 0.0, fz_cur = 0.0;
     // warm-up: planes z0-1, z0 of u,v,w; phi planes z0-2..z0 and fz(z0-1)
@@ -139,83 +223,78 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     lds_sync();
     fetch(z0 + 1);
 
-    for (int k = z0; k < z1; k++) {
+    double dprev = 0.0, wsprev = 0.0;   // divergence partial and w* of the previous plane
+    for (int k = z0; k <= z1; k++) {
+        const bool last = k == z1;      // one plane past the chunk: w* only
+        const int kg = last ? wrapn(k, n) : k;
         // AB2 history of the own column: issued before the barriers
-        const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
-        double rou = 0.0, rov = 0.0, row = 0.0;
-        if (live) { rou = ru_o[q]; rov = rv_o[q]; row = rw_o[q]; }
+        const size_t q = idx3(ic, jc, kg, n);
+        const double row = rw_o[q];
+        double rou = 0.0, rov = 0.0;
+        if (!last) { rou = ru_o[q]; rov = rv_o[q]; }
+        // the cell just past the tile, for the edge waves: x = MX (wave 0),
+        // y = MY (wave 1)
+        const bool xedge = tid < MY, yedge = tid >= 64 && tid < 64 + MX;
+        const int eb = tid, ea = tid - 64;
         commit(k + 1);
         lds_sync();
         if (PHI) {
-            // S1 at plane k: fx on x = -1..MX-1, fy on y = -1..MY-1, fz on the column
+            // S1 at plane k: fx on x = -1..MX, fy on y = -1..MY, fz on the column
             const double c0 = chem(k, tx, ty);
-            sfx[ty * (MX + 1) + tx + 1] = sig6 * c0 * 0.5 * (P(k, tx + 1, ty) - P(k, tx - 1, ty)) * dxi;
-            sfy[(ty + 1) * MX + tx] = sig6 * c0 * 0.5 * (P(k, tx, ty + 1) - P(k, tx, ty - 1)) * dxi;
             fz_cur = sig6 * c0 * 0.5 * (P(k + 1, tx, ty) - P(k - 1, tx, ty)) * dxi;
-            if (tid < MY) {            // x = -1 column
-                const int b = tid;
-                sfx[b * (MX + 1)] = sig6 * chem(k, -1, b) * 0.5 * (P(k, 0, b) - P(k, -2, b)) * dxi;
-            } else if (tid < MY + MX) { // y = -1 row
-                const int a = tid - MY;
-                sfy[a] = sig6 * chem(k, a, -1) * 0.5 * (P(k, a, 0) - P(k, a, -2)) * dxi;
+            if (!last) {
+                sfx[ty * FXW + tx + 1] = sig6 * c0 * 0.5 * (P(k, tx + 1, ty) - P(k, tx - 1, ty)) * dxi;
+                sfy[(ty + 1) * MX + tx] = sig6 * c0 * 0.5 * (P(k, tx, ty + 1) - P(k, tx, ty - 1)) * dxi;
+                if (tid < 2 * MY) {                 // x = -1 and x = MX columns
+                    const int b = tid >> 1, a = (tid & 1) ? MX : -1;
+                    sfx[b * FXW + a + 1] = sig6 * chem(k, a, b) * 0.5 * (P(k, a + 1, b) - P(k, a - 1, b)) * dxi;
+                } else if (tid < 2 * MY + 2 * MX) { // y = -1 and y = MY rows
+                    const int e = tid - 2 * MY;
+                    const int a = e >> 1, b = (e & 1) ? MY : -1;
+                    sfy[(b + 1) * MX + a] = sig6 * chem(k, a, b) * 0.5 * (P(k, a, b + 1) - P(k, a, b - 1)) * dxi;
+                }
             }
             lds_sync();
         }
-        if (k + 2 <= z1) fetch(k + 2);   // next plane in flight during the compute
+        if (k + 2 <= z1 + 1) fetch(k + 2);   // next plane in flight during the compute
         const int sm = ring3(k - 1), sc = ring3(k), sq = ring3(k + 1);
-        const int c = (ty + 1) * W1 + (tx + 1);
-        const double u0 = su[sc][c], v0 = sv[sc][c], w0 = sw[sc][c];
-        const double uxp = su[sc][c + 1], uxm = su[sc][c - 1], uyp = su[sc][c + W1], uym = su[sc][c - W1];
-        const double uzp = su[sq][c], uzm = su[sm][c];
-        const double vxp = sv[sc][c + 1], vxm = sv[sc][c - 1], vyp = sv[sc][c + W1], vym = sv[sc][c - W1];
-        const double vzp = sv[sq][c], vzm = sv[sm][c];
-        const double wxp = sw[sc][c + 1], wxm = sw[sc][c - 1], wyp = sw[sc][c + W1], wym = sw[sc][c - W1];
-        const double wzp = sw[sq][c], wzm = sw[sm][c];
-        double h11 = (uxp + u0) * (uxp + u0) - (u0 + uxm) * (u0 + uxm);
-        double h12 = (uyp + u0) * (vyp + sv[sc][c + W1 - 1]) - (u0 + uym) * (v0 + vxm);
-        double h13 = (uzp + u0) * (wzp + sw[sq][c - 1]) - (u0 + uzm) * (w0 + wxm);
-        double h21 = (uxp + su[sc][c + 1 - W1]) * (vxp + v0) - (u0 + uym) * (v0 + vxm);
-        double h22 = (vyp + v0) * (vyp + v0) - (v0 + vym) * (v0 + vym);
-        double h23 = (wzp + sw[sq][c - W1]) * (vzp + v0) - (w0 + wym) * (v0 + vzm);
-        double h31 = (wxp + w0) * (uxp + su[sm][c + 1]) - (w0 + wxm) * (u0 + uzm);
-        double h32 = (vyp + sv[sm][c + W1]) * (wyp + w0) - (v0 + vzm) * (w0 + wym);
-        double h33 = (wzp + w0) * (wzp + w0) - (w0 + wzm) * (w0 + wzm);
-        h11 = h11 * 0.25 * dxi; h12 = h12 * 0.25 * dxi; h13 = h13 * 0.25 * dxi;
-        h21 = h21 * 0.25 * dxi; h22 = h22 * 0.25 * dxi; h23 = h23 * 0.25 * dxi;
-        h31 = h31 * 0.25 * dxi; h32 = h32 * 0.25 * dxi; h33 = h33 * 0.25 * dxi;
-        double ru = -(h11 + h12 + h13);
-        double rv = -(h21 + h22 + h23);
-        double rw = -(h31 + h32 + h33);
-        h11 = mu * (uxp - 2.0 * u0 + uxm) * ddxi;
-        h12 = mu * (uyp - 2.0 * u0 + uym) * ddxi;
-        h13 = mu * (uzp - 2.0 * u0 + uzm) * ddxi;
-        h21 = mu * (vxp - 2.0 * v0 + vxm) * ddxi;
-        h22 = mu * (vyp - 2.0 * v0 + vym) * ddxi;
-        h23 = mu * (vzp - 2.0 * v0 + vzm) * ddxi;
-        h31 = mu * (wxp - 2.0 * w0 + wxm) * ddxi;
-        h32 = mu * (wyp - 2.0 * w0 + wym) * ddxi;
-        h33 = mu * (wzp - 2.0 * w0 + wzm) * ddxi;
-        ru = ru + (h11 + h12 + h13) * rhoi;
-        rv = rv + (h21 + h22 + h23) * rhoi;
-        rw = rw + (h31 + h32 + h33) * rhoi;
-        if (live) {
-            ru = ru + g.f3 * mysin[k + 1] + g.f2 * mycos[j + 1];
-            rv = rv + g.f1 * mysin[i] + g.f3 * mycos[k + 1];
-            rw = rw + g.f2 * mysin[j + 1] + g.f1 * mycos[i];
-            if (PHI) {
-                ru = ru + 0.5 * (sfx[ty * (MX + 1) + tx] + sfx[ty * (MX + 1) + tx + 1]) * rhoi;
-                rv = rv + 0.5 * (sfy[ty * MX + tx] + sfy[(ty + 1) * MX + tx]) * rhoi;
-                rw = rw + 0.5 * (fz_prev + fz_cur) * rhoi;
-            }
-            us[q] = u0 + dt * (alpha * ru - beta * rou);
-            vs[q] = v0 + dt * (alpha * rv - beta * rov);
-            ws[q] = w0 + dt * (alpha * rw - beta * row);
-            ru_o[q] = ru;
-            rv_o[q] = rv;
-            rw_o[q] = rw;
+        const int c = at(tx, ty);
+        const double rw = rhs_w(sm, sc, sq, fz_prev, fz_cur);
+        const double wsk = sw[sc][c] + dt * (alpha * rw - beta * row);
+        if (live && k > z0) {
+            // main.f90:841-844: rhsp = rho/(dt dx) sum_d (u*_d(+1) - u*_d)
+            const double r = dprev + cdiv * (wsk - wsprev);
+            div[idx3(i, j, k - 1, n)] = (TD)r;
         }
+        double usk = 0.0, vsk = 0.0;
+        if (!last) {
+            if (live) { ws[q] = wsk; rw_n[q] = rw; }
+            const double ru = rhs_u(sm, sc, sq, tx, ty, jc, k);
+            usk = su[sc][c] + dt * (alpha * ru - beta * rou);
+            if (live) { us[q] = usk; ru_n[q] = ru; }
+            const double rv = rhs_v(sm, sc, sq, tx, ty, ic, k);
+            vsk = sv[sc][c] + dt * (alpha * rv - beta * rov);
+            if (live) { vs[q] = vsk; rv_n[q] = rv; }
+            sus[ty * (MX + 1) + tx] = usk;
+            svs[ty * MX + tx] = vsk;
+            if (xedge) {
+                const double roe = ru_o[idx3(wrapn(i0 + MX, n), wrapn(j0 + eb, n), k, n)];
+                const double re = rhs_u(sm, sc, sq, MX, eb, wrapn(j0 + eb, n), k);
+                sus[eb * (MX + 1) + MX] = su[sc][at(MX, eb)] + dt * (alpha * re - beta * roe);
+            } else if (yedge) {
+                const double roe = rv_o[idx3(wrapn(i0 + ea, n), wrapn(j0 + MY, n), k, n)];
+                const double re = rhs_v(sm, sc, sq, ea, MY, wrapn(i0 + ea, n), k);
+                svs[MY * MX + ea] = sv[sc][at(ea, MY)] + dt * (alpha * re - beta * roe);
+            }
+        }
+        wsprev = wsk;
         fz_prev = fz_cur;
         lds_sync();
+        if (!last) {
+            double r = cdiv * (sus[ty * (MX + 1) + tx + 1] - usk);
+            r = r + cdiv * (svs[(ty + 1) * MX + tx] - vsk);
+            dprev = r;
+        }
     }
 }
 
@@ -407,16 +486,31 @@ int launch_ns_march(mhit_ctx* c) {
     const int n = c->n;
     const int zc = n < c->tune.ns_mz ? n : c->tune.ns_mz;
     dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, zc)), block(NS_MX, NS_MY, 1);
-    if (c->P.phiflag)
-        k_ns_march<true><<<grid, block, 0, c->stream>>>(
-            c->u, c->v, c->w, c->phi, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
-            c->mysin, c->mycos, c->g, c->S, zc, c->tune.xcd);
-    else
-        k_ns_march<false><<<grid, block, 0, c->stream>>>(
-            c->u, c->v, c->w, nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->us, c->vs, c->ws,
-            c->mysin, c->mycos, c->g, c->S, zc, c->tune.xcd);
+    const double cdiv = c->g.rho * c->g.dxi / c->g.dt;
+    auto go = [&](auto phi_tag, auto* div) {
+        constexpr bool PH = decltype(phi_tag)::value;
+        using TD = std::remove_pointer_t<decltype(div)>;
+        k_ns_march<PH, TD><<<grid, block, 0, c->stream>>>(
+            c->u, c->v, c->w, PH ? c->phi : nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->rhsu_n,
+            c->rhsv_n, c->rhsw_n, c->us, c->vs, c->ws, div, cdiv, c->mysin, c->mycos, c->g, c->S, zc,
+            c->tune.xcd);
+    };
+    // the divergence goes to the pressure buffer, which the x R2C pass reads
+    // and the x C2R pass overwrites with p
+    if (c->P.phiflag) {
+        if (c->fp32) go(std::true_type{}, (float*)c->pres);
+        else go(std::true_type{}, (double*)c->pres);
+    } else {
+        if (c->fp32) go(std::false_type{}, (float*)c->pres);
+        else go(std::false_type{}, (double*)c->pres);
+    }
     prof_mark(c, ST_NS);
     MHIT_HIP_TRY(hipGetLastError());
+    // the new AB2 history is in the other buffers (neighbours' old values
+    // are read during the pass)
+    std::swap(c->rhsu_o, c->rhsu_n);
+    std::swap(c->rhsv_o, c->rhsv_n);
+    std::swap(c->rhsw_o, c->rhsw_n);
     return MHIT_OK;
 }
 

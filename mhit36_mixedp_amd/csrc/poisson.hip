@@ -28,10 +28,14 @@ namespace mhit {
 // One workgroup: PX pairs of y rows at one z.  Row pair (ya, ya+1) is
 // transformed as one complex sequence a + i b and split with
 // X_a[k] = (Z[k] + conj Z[N-k])/2, X_b[k] = (Z[k] - conj Z[N-k])/(2i).
-template <typename T, int N, bool DIV>
+// Input IN: X_IN_FIELD an FP64 field (us), X_IN_DIV the divergence of
+// (us, vs, ws), X_IN_READY a field already in the transform precision
+// (din: the divergence written by the fused NS pass).
+enum { X_IN_FIELD = 0, X_IN_DIV = 1, X_IN_READY = 2 };
+template <typename T, int N, int IN>
 __global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
 void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
-                 const double* __restrict__ ws, double cdiv,
+                 const double* __restrict__ ws, const T* __restrict__ din, double cdiv,
                  cplx_t<T>* __restrict__ spec, cplx_t<T>* __restrict__ nyq,
                  const cplx_t<T>* __restrict__ tw, const DevScalars* __restrict__ S) {
     using V = cplx_t<T>;
@@ -54,12 +58,19 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
         const int e = tid + r * NT;
         if ((N * P) % NT != 0 && e >= N * P) break;
         const int a = e % N, c = e / N;
+        V val;
+        if (IN == X_IN_READY) {
+            val.x = din[idx3(a, y0 + 2 * c, z, N)];
+            val.y = din[idx3(a, y0 + 2 * c + 1, z, N)];
+            lds[a * LD + c] = val;
+            continue;
+        }
         double d[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int y = y0 + 2 * c + h;
             const size_t q = idx3(a, y, z, N);
-            if (DIV) {
+            if (IN == X_IN_DIV) {
                 // rhsp, main.f90:841-843
                 const int ap = (a + 1 == N) ? 0 : a + 1;
                 const int yp = (y + 1 == N) ? 0 : y + 1;
@@ -71,7 +82,6 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
                 d[h] = us[q];
             }
         }
-        V val;
         val.x = (T)d[0];   // rhspp = real(rhsp,4) in the FP32 form (main.f90:844)
         val.y = (T)d[1];
         lds[a * LD + c] = val;
@@ -319,15 +329,19 @@ struct PoissonPasses {
     using Cfg = FftCfg<T, N>;
     static constexpr int H = N / 2;
 
-    static void x_fwd(mhit_ctx* c, const double* a, const double* b, const double* w, bool div) {
+    static void x_fwd(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
         dim3 grid(N / (2 * Cfg::PX), N), block(Cfg::PX * Cfg::TPC);
         const double cdiv = c->g.rho * c->g.dxi / c->g.dt;
-        if (div)
-            k_fft_x_fwd<T, N, true><<<grid, block, 0, c->stream>>>(
-                a, b, w, cdiv, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
+        if (in == X_IN_READY)
+            k_fft_x_fwd<T, N, X_IN_READY><<<grid, block, 0, c->stream>>>(
+                nullptr, nullptr, nullptr, (const T*)c->pres, 0.0, (V*)c->spec, (V*)c->nyq,
+                (const V*)c->tw, c->S);
+        else if (in == X_IN_DIV)
+            k_fft_x_fwd<T, N, X_IN_DIV><<<grid, block, 0, c->stream>>>(
+                a, b, w, nullptr, cdiv, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
         else
-            k_fft_x_fwd<T, N, false><<<grid, block, 0, c->stream>>>(
-                a, nullptr, nullptr, 0.0, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
+            k_fft_x_fwd<T, N, X_IN_FIELD><<<grid, block, 0, c->stream>>>(
+                a, nullptr, nullptr, nullptr, 0.0, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
     }
     template <int MODE>
     static void y_pass(mhit_ctx* c) {
@@ -343,7 +357,10 @@ struct PoissonPasses {
     }
     template <int MODE>
     static void z_pass(mhit_ctx* c) {
-        constexpr int CY = Cfg::CY;
+        // half the y pass's columns per workgroup: the two FFTs of the z pass
+        // make it the longest-running column kernel, and smaller workgroups
+        // (two per CU at its register count) hide its barriers better
+        constexpr int CY = Cfg::CY / 2 > 0 ? Cfg::CY / 2 : 1;
         const double ntot = (double)((long long)N * N * N);
         // spec: columns kx at fixed y (blockIdx.y); axis z (stride N*H)
         k_fft_cols<T, N, CY, false, MODE, false>
@@ -359,8 +376,8 @@ struct PoissonPasses {
         k_fft_x_inv<T, N><<<grid, block, 0, c->stream>>>(
             (const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S);
     }
-    static void project(mhit_ctx* c, const double* a, const double* b, const double* w, bool div) {
-        x_fwd(c, a, b, w, div);
+    static void project(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
+        x_fwd(c, a, b, w, in);
         prof_mark(c, ST_FFT_X_FWD);
         y_pass<0>(c);
         prof_mark(c, ST_FFT_Y_FWD);
@@ -372,7 +389,7 @@ struct PoissonPasses {
         prof_mark(c, ST_FFT_X_INV);
     }
     static void r2c(mhit_ctx* c, const double* in) {
-        x_fwd(c, in, nullptr, nullptr, false);
+        x_fwd(c, in, nullptr, nullptr, X_IN_FIELD);
         y_pass<0>(c);
         z_pass<0>(c);
     }
@@ -395,15 +412,17 @@ bool fft_size_supported(int n) {
     }
 }
 
-enum class Op { Project, Solve, R2C, C2R };
+enum class Op { Project, ProjectReady, Solve, R2C, C2R };
 
 template <typename T>
 static int dispatch(mhit_ctx* c, Op op, const double* a, const double* b, const double* w) {
     switch (c->n) {
 #define CASE(NN)                                                                   \
     case NN:                                                                       \
-        if (op == Op::Project) PoissonPasses<T, NN>::project(c, a, b, w, true);   \
-        else if (op == Op::Solve) PoissonPasses<T, NN>::project(c, a, b, w, false);\
+        if (op == Op::Project) PoissonPasses<T, NN>::project(c, a, b, w, X_IN_DIV);  \
+        else if (op == Op::ProjectReady)                                          \
+            PoissonPasses<T, NN>::project(c, a, b, w, X_IN_READY);                \
+        else if (op == Op::Solve) PoissonPasses<T, NN>::project(c, a, b, w, X_IN_FIELD);\
         else if (op == Op::R2C) PoissonPasses<T, NN>::r2c(c, a);                  \
         else PoissonPasses<T, NN>::c2r(c);                                         \
         break;
@@ -421,7 +440,12 @@ static int run(mhit_ctx* c, Op op, const double* a, const double* b, const doubl
     return c->fp32 ? dispatch<float>(c, op, a, b, w) : dispatch<double>(c, op, a, b, w);
 }
 
-int poisson_project(mhit_ctx* c) { return run(c, Op::Project, c->us, c->vs, c->ws); }
+// The fused NS pass leaves the divergence of u* in the pressure buffer; the
+// cell-per-thread path leaves u* only.
+int poisson_project(mhit_ctx* c) {
+    return c->march ? run(c, Op::ProjectReady, nullptr, nullptr, nullptr)
+                    : run(c, Op::Project, c->us, c->vs, c->ws);
+}
 int poisson_solve_field(mhit_ctx* c, const double* rhs) { return run(c, Op::Solve, rhs, nullptr, nullptr); }
 int fft_r2c_dev(mhit_ctx* c, const double* in) { return run(c, Op::R2C, in, nullptr, nullptr); }
 int fft_c2r_dev(mhit_ctx* c) { return run(c, Op::C2R, nullptr, nullptr, nullptr); }
