@@ -207,6 +207,8 @@ def main():
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes instead of mhit_autotune's choice")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -228,6 +230,10 @@ def main():
     copy_gbps = s.copy_probe(1 << 30, 10) if not args.no_copy_probe else None
     if copy_gbps:
         log(f"[bench] plain device copy: {copy_gbps:.0f} GB/s (read + write)")
+    tuning = None
+    if not args.no_autotune:
+        tuning = s.autotune(steps=3)          # set-up, before the timed region
+        log(f"[bench] autotune: {tuning}")
     ms32 = time_steps(s, args.warmup, args.steps)
     log(f"[bench] FP32 Poisson: {ms32:.3f} ms/step")
     s.profile_enable(True)
@@ -247,6 +253,8 @@ def main():
     prof64 = None
     if not args.no_fp64:
         s = make_solver(args, prec=0)
+        if not args.no_autotune:
+            log(f"[bench] autotune (FP64 Poisson): {s.autotune(steps=3)}")
         ms64 = time_steps(s, args.warmup, args.steps)
         log(f"[bench] FP64 Poisson: {ms64:.3f} ms/step")
         s.profile_enable(True)
@@ -326,6 +334,7 @@ def main():
             "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
+        "tuning": tuning,
         "ensemble": ens,
         "small_box": small,
         "cpu_baseline": cpu,
