@@ -3,6 +3,8 @@
 #include "fft.hpp"
 #include "internal.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -329,6 +331,11 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     // MHIT_STENCIL_PATH=cell selects the one-cell-per-thread kernels that
     // follow the reference's loop split (a cross-check of the fused passes)
     if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
+    // default z chunks by grid size, n/8 within [8, 64] (measured: 64 planes
+    // for the 512^3 box; 8 for the 64^3 ensemble boxes, which need many
+    // workgroups per box); mhit_autotune / MHIT_TUNE refine
+    c->tune.ns_mz = std::min(64, std::max(8, n / 8));
+    c->tune.ac_mz = std::min(64, std::max(8, n / 8));
     if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
     // MHIT_GRAPH=0 launches the kernels of every step one by one
     if (const char* gr = std::getenv("MHIT_GRAPH")) c->use_graph = std::strcmp(gr, "0") != 0;

@@ -43,8 +43,8 @@ struct DevScalars {
 // marching passes, XCD-aware tile order).  Defaults are the measured best
 // at 512^3; MHIT_TUNE="ns_mz=64,ac_mz=64,xcd=1,sort=10" overrides them.
 struct Tuning {
-    int ns_mz = 64;       // planes per workgroup, NS pass
-    int ac_mz = 64;       // planes per workgroup, ACDI pass
+    int ns_mz = 64;       // planes per workgroup, NS pass (mhit_create: n/8 in [8, 64])
+    int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
     int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
 };

@@ -242,10 +242,14 @@ int launch_ns(mhit_ctx* c) {
 // fetched into registers during the update.  The number of partials is
 // fixed by the grid, so the reduction order is deterministic.
 constexpr int kCorrZ = 16;                    // z split of the diagnostics pass
-constexpr int CO_MX = 64, CO_MY = 8, CO_MZ = 32;
+constexpr int CO_MX = 64, CO_MY = 8;
+// z planes per workgroup: a function of n only (the number of partials, and
+// so the reduction order, is fixed per grid size); small boxes get short
+// chunks so that a box still spreads over many workgroups
+static int correct_zc(int n) { return n >= 256 ? 32 : (n >= 128 ? 16 : 8); }
 
 static dim3 correct_grid(int n) {
-    const int zc = n < CO_MZ ? n : CO_MZ;
+    const int zc = correct_zc(n) < n ? correct_zc(n) : n;
     return dim3(div_up(n, CO_MX), div_up(n, CO_MY), div_up(n, zc));
 }
 
@@ -336,7 +340,7 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
 
 int launch_correct(mhit_ctx* c) {
     const int n = c->n;
-    const int zc = n < CO_MZ ? n : CO_MZ;
+    const int zc = correct_zc(n) < n ? correct_zc(n) : n;
     dim3 grid = correct_grid(n), block(CO_MX, CO_MY, 1);
     if (c->fp32)
         k_correct<float><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const float*)c->pres,

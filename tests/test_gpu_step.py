@@ -159,3 +159,42 @@ def test_multiphase_partial_tiles_and_chunks(n, oracle_mod):
     e = compare(s, o, phi=True, tr=True)
     print(e, flush=True)
     assert max(e.values()) < 1e-11, e
+
+
+@pytest.mark.parametrize("interp", [0, 1], ids=["reference", "trilinear"])
+def test_tracers_on_cell_faces_and_domain_edges(oracle_mod, interp):
+    """Tracers placed exactly on grid lines, at 0 and just below 2 pi on each
+    axis (the wrap of the interpolation, particles.f90:78-122, and of the
+    position, main.f90:501-508), stepped against the oracle."""
+    print(f"[tracers] edges interp={interp}", flush=True)
+    n = 24
+    L = 2 * np.pi
+    dx = L / n
+    edge = [0.0, dx, 0.5 * dx, L - 1e-13, L - dx, L - 0.5 * dx, 3 * dx, L / 2]
+    pts = np.array([(a, b, c) for a in edge for b in edge[:4] for c in edge[2:6]])
+    s, o = make_pair(oracle_mod, n, 0, partflag=1, interp_mode=interp)
+    s.upload_tracers(pts[:, 0], pts[:, 1], pts[:, 2])
+    o.set_tracers(pts[:, 0], pts[:, 1], pts[:, 2])
+    for _ in range(3):
+        s.step(1)
+        assert o.step(1) == 0
+    e = compare(s, o, tr=True)
+    t = s.download_tracers()
+    assert np.all((t["x"] >= 0) & (t["x"] < L)) and np.all((t["z"] >= 0) & (t["z"] < L))
+    assert max(e.values()) < 1e-12, e
+    s.close()
+    o.close()
+
+
+def test_tracer_flag_with_no_tracers(oracle_mod):
+    """partflag = 1 with an empty tracer set steps like partflag = 0."""
+    print("[tracers] none", flush=True)
+    s, o = make_pair(oracle_mod, 16, 0, partflag=1)
+    s.upload_tracers(np.zeros(0), np.zeros(0), np.zeros(0))
+    assert s.download_tracers()["x"].size == 0
+    s.step(3)
+    assert o.step(3) == 0
+    e = compare(s, o)
+    assert max(e.values()) < 1e-12, e
+    s.close()
+    o.close()
