@@ -191,7 +191,20 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
     __shared__ V stw[N];
     if (S && S->halted) return;
     const int tid = threadIdx.x;
-    V* __restrict__ g = data + blockIdx.x * bx + blockIdx.y * by;
+    // Workgroups dispatch to the 8 XCDs round-robin, each XCD with its own
+    // L2.  When a row segment is half a 128-byte line, the two workgroups
+    // that share the line are renumbered to land on the same XCD, 8
+    // dispatches apart, so the second reads the line from L2.
+    int bxi = blockIdx.x, byi = blockIdx.y;
+    if (!AXIS_CONTIG && C * (int)sizeof(V) == 64 && gridDim.x % 2 == 0 &&
+        (gridDim.x * gridDim.y) % 16 == 0) {
+        const int L = blockIdx.x + gridDim.x * blockIdx.y;
+        const int r = L % 16;
+        const int t = L - r + (r % 8) * 2 + r / 8;
+        bxi = t % gridDim.x;
+        byi = t / gridDim.x;
+    }
+    V* __restrict__ g = data + bxi * bx + byi * by;
 
     static_assert((N * C) % NT == 0, "column tile must be a multiple of the workgroup");
     // Register-direct form (strided columns, N/8 threads per column, radix-8
@@ -232,8 +245,8 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             // One FP64 reciprocal per element replaces the reference's two
             // divisions per component (at most 2 ulp apart in FP64).
             __syncthreads();   // every thread holds its last-stage inputs
-            const int kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
-            const int kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+            const int kxi = NYQ ? N / 2 : bxi * C + c;
+            const int kyi = NYQ ? bxi * C + c : byi;
             const double kxy = skk[kxi] + skk[kyi];
 #pragma unroll
             for (int r = 0; r < 8; r++) {
@@ -280,8 +293,8 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
     double kxy = 0.0;
     int kxi = 0, kyi = 0;
     if constexpr (MODE == 2) {
-        kxi = NYQ ? N / 2 : (int)blockIdx.x * C + c;
-        kyi = NYQ ? (int)blockIdx.x * C + c : (int)blockIdx.y;
+        kxi = NYQ ? N / 2 : bxi * C + c;
+        kyi = NYQ ? bxi * C + c : byi;
         kxy = kk[kxi] + kk[kyi];
 #pragma unroll
         for (int q = 0; q < NA; q++) {
