@@ -207,8 +207,9 @@ def main():
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="keep the default launch shapes instead of mhit_autotune's choice")
+    ap.add_argument("--autotune", action="store_true",
+                    help="let mhit_autotune pick the launch shapes before timing (default: the "
+                         "library's size-based defaults, which the kernel trace under profiles/ uses)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -230,10 +231,10 @@ def main():
     copy_gbps = s.copy_probe(1 << 30, 10) if not args.no_copy_probe else None
     if copy_gbps:
         log(f"[bench] plain device copy: {copy_gbps:.0f} GB/s (read + write)")
-    tuning = None
-    if not args.no_autotune:
-        tuning = s.autotune(steps=3)          # set-up, before the timed region
-        log(f"[bench] autotune: {tuning}")
+    s_tuning = s.tuning()                     # the shapes timed below
+    if args.autotune:
+        s_tuning = s.autotune(steps=3)        # set-up, before the timed region
+        log(f"[bench] autotune: {s_tuning}")
     ms32 = time_steps(s, args.warmup, args.steps)
     log(f"[bench] FP32 Poisson: {ms32:.3f} ms/step")
     s.profile_enable(True)
@@ -253,7 +254,7 @@ def main():
     prof64 = None
     if not args.no_fp64:
         s = make_solver(args, prec=0)
-        if not args.no_autotune:
+        if args.autotune:
             log(f"[bench] autotune (FP64 Poisson): {s.autotune(steps=3)}")
         ms64 = time_steps(s, args.warmup, args.steps)
         log(f"[bench] FP64 Poisson: {ms64:.3f} ms/step")
@@ -334,7 +335,7 @@ def main():
             "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
-        "tuning": tuning,
+        "tuning": s_tuning,
         "ensemble": ens,
         "small_box": small,
         "cpu_baseline": cpu,
