@@ -145,8 +145,10 @@ ENSEMBLE_SIZES = (64, 96, 128)
 def ensemble_leg(args):
     """Ensemble workload: 16 boxes each of 64^3, 96^3 and 128^3 (51,904,512
     cells), every other box multiphase, each with its own seed, viscosity
-    and drop count.  Returns box-steps/s stepped together (one stream per
-    box) and stepped one box at a time, with the algorithmic HBM rate."""
+    and drop count.  Returns box-steps/s stepped together as one captured
+    graph per step -- batched (boxes of one size share every launch) and
+    with one graph branch per box -- and stepped one box at a time, with
+    the algorithmic HBM rate of the batched form."""
     from mhit36_mixedp_amd import Ensemble, default_params
     params, phase = [], []
     for i in range(16 * len(ENSEMBLE_SIZES)):
@@ -168,6 +170,12 @@ def ensemble_leg(args):
         ens.step(args.ensemble_steps)
         t1 = time.perf_counter()
         together = (t1 - t0) / args.ensemble_steps          # s per ensemble step
+        ens.set_batched(False)
+        ens.step(args.warmup)
+        t0 = time.perf_counter()
+        ens.step(args.ensemble_steps)
+        t1 = time.perf_counter()
+        branches = (t1 - t0) / args.ensemble_steps
         t0 = time.perf_counter()
         for b in ens.boxes:
             b.step(args.ensemble_steps)
@@ -183,6 +191,7 @@ def ensemble_leg(args):
         "box_steps_per_s": round(nbox / together, 1),
         "ms_per_ensemble_step": round(together * 1e3, 4),
         "hbm_GBps": round(nbytes / together / 1e9, 1),
+        "branch_per_box_box_steps_per_s": round(nbox / branches, 1),
         "one_at_a_time_box_steps_per_s": round(nbox / alone, 1),
     }
 

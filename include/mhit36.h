@@ -253,13 +253,17 @@ int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
  * box per job; this has no reference counterpart (BASELINE ensemble
  * configuration).  Each box is a full context: initialise, upload and
  * download through mhit_ensemble_box().  mhit_ensemble_step advances every
- * box nsteps steps, boxes running concurrently on their own streams;
- * returns MHIT_ERR_CFL if any box stopped on cou > 7 (that box stays
- * frozen, the others keep stepping).                                     */
+ * box nsteps steps as one captured graph; returns MHIT_ERR_CFL if any box
+ * stopped on cou > 7 (that box stays frozen, the others keep stepping).
+ * mhit_ensemble_set_batched: on (the default), boxes of one grid size share
+ * every launch of a pass (one launch per pass and size); off, each box
+ * runs its own launches on a graph branch of its own.  Results are the
+ * same bits either way, and the same as each box stepped alone.           */
 typedef struct mhit_ensemble mhit_ensemble;
 int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble** out);
 int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);
 int mhit_ensemble_step(mhit_ensemble* ens, int nsteps);
+int mhit_ensemble_set_batched(mhit_ensemble* ens, int32_t on);
 int mhit_ensemble_destroy(mhit_ensemble* ens);
 
 /* ---- per-stage timing ------------------------------------------------ */

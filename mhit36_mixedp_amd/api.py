@@ -274,10 +274,12 @@ class Ensemble:
     """Independent boxes advanced together on one GPU (mhit_ensemble_*).
 
     ``boxes[i]`` is a Solver view of box i (initialise, upload, download,
-    diagnose through it); ``step(n)`` advances every box n steps with the
-    boxes running concurrently."""
+    diagnose through it); ``step(n)`` advances every box n steps as one
+    captured graph per step.  ``batched`` (default True): boxes of one grid
+    size share every launch of a pass; False: one graph branch per box.
+    The results are the same bits either way."""
 
-    def __init__(self, params_list):
+    def __init__(self, params_list, batched: bool = True):
         self.lib = _abi.load_library()
         arr = (Params * len(params_list))(*params_list)
         h = ctypes.c_void_p()
@@ -289,6 +291,11 @@ class Ensemble:
             b = ctypes.c_void_p()
             _check(self.lib.mhit_ensemble_box(h, i, ctypes.byref(b)), "mhit_ensemble_box")
             self.boxes.append(Solver(p, _handle=b))
+        if not batched:
+            self.set_batched(False)
+
+    def set_batched(self, on: bool):
+        _check(self.lib.mhit_ensemble_set_batched(self._h, int(bool(on))), "mhit_ensemble_set_batched")
 
     def step(self, nsteps: int = 1):
         _check(self.lib.mhit_ensemble_step(self._h, int(nsteps)), "mhit_ensemble_step")

@@ -81,10 +81,62 @@ struct CellLaunch {
 // of neighbouring tiles whose halo rows and columns then come from that
 // XCD's L2.  Identity when the tile count is not a multiple of 8.
 constexpr int kXcds = 8;
-__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, int enable = 1) {
-    const int nx = gridDim.x, ny = gridDim.y, nz = gridDim.z;
+
+// Workgroup coordinates within one box.  A launch may cover several boxes
+// of the same size (an ensemble): their grids are stacked along z, so
+// gridDim.z = nz * nbox and box = blockIdx.z / nz.  A single box is nbox = 1.
+// Because each box's grid is a whole multiple of the per-box workgroup
+// count, the XCD that a workgroup lands on (linear id mod 8) is the same as
+// in a launch of that box alone whenever that count is a multiple of 8.
+struct Blk {
+    int x, y, z;      // workgroup coordinates in the box's grid
+    int nx, ny, nz;   // the box's grid
+    int box;          // index of the box in the launch
+};
+__device__ __forceinline__ Blk launch_blk(int nbox) {
+    Blk b;
+    b.x = blockIdx.x;
+    b.y = blockIdx.y;
+    b.nx = gridDim.x;
+    b.ny = gridDim.y;
+    b.nz = gridDim.z / nbox;
+    b.box = blockIdx.z / b.nz;
+    b.z = blockIdx.z - b.box * b.nz;
+    return b;
+}
+
+// Device pointers taken from an argument struct (possibly read from a table
+// in memory) are generic to the compiler, which then emits flat memory
+// instructions; gmem() states that they point to global memory.
+#define MHIT_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ MHIT_GLOBAL T* gmem(T* p) { return (MHIT_GLOBAL T*)p; }
+
+// Load / store of a two-component HIP vector (float2, double2) through a
+// global pointer: the vector type's operators take generic references, so
+// the pair moves as a builtin two-element vector.
+template <typename V>
+__device__ __forceinline__ V gload(const MHIT_GLOBAL V* p) {
+    typedef typename V::value_type E __attribute__((ext_vector_type(2)));
+    const E t = *(const MHIT_GLOBAL E*)p;
+    V r;
+    r.x = t.x;
+    r.y = t.y;
+    return r;
+}
+template <typename V>
+__device__ __forceinline__ void gstore(MHIT_GLOBAL V* p, const V& v) {
+    typedef typename V::value_type E __attribute__((ext_vector_type(2)));
+    E t;
+    t.x = v.x;
+    t.y = v.y;
+    *(MHIT_GLOBAL E*)p = t;
+}
+
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, const Blk& b, int enable = 1) {
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
     const int total = nx * ny * nz;
-    int L = blockIdx.x + nx * (blockIdx.y + ny * blockIdx.z);
+    int L = b.x + nx * (b.y + ny * b.z);
     if (enable && total % kXcds == 0) L = (L % kXcds) * (total / kXcds) + L / kXcds;
     bx = L % nx;
     by = (L / nx) % ny;

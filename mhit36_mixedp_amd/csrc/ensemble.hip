@@ -1,18 +1,24 @@
 // ensemble.hip -- many independent boxes advanced together on one GPU.
 //
-// Each box is a full context.  One ensemble timestep is ONE captured graph
-// whose branches are the boxes' timesteps: the capture forks from the
-// ensemble stream into every box's stream and joins back, so the replay
-// lets the boxes' kernels overlap on the GPU (small boxes cannot fill 256
-// CUs alone) at the cost of a single launch.  The tracer re-ordering
-// (hipCUB radix sort, every sort_every steps) runs eagerly on the ensemble
-// stream before the replay.  Graphs are keyed by every box's current
-// buffers (the ping-pong parities, tracer columns).  Boxes share nothing; a
-// box stopped by the CFL check stays frozen (its kernels return at once)
-// while the others continue.
+// Each box is a full context.  One ensemble timestep is ONE captured graph.
+// By default (batched) the boxes are grouped by grid size, one graph branch
+// per size; within a branch every pass is ONE launch over all boxes of that
+// size (their grids stacked along z, each workgroup reading its box's
+// arguments from a table), so small boxes fill the 256 CUs together and a
+// timestep is a few dozen launches instead of ten per box.  Unbatched, the
+// branches are the boxes' own timesteps (the capture forks from the
+// ensemble stream into every box's stream and joins back).  The tracer
+// re-ordering (hipCUB radix sort, every sort_every steps) runs eagerly on
+// the ensemble stream before the replay.  Graphs are keyed by every box's
+// current buffers (the ping-pong parities, tracer columns).  Boxes share
+// nothing; a box stopped by the CFL check stays frozen (its kernels return
+// at once) while the others continue.  Both forms compute every box with
+// the same kernels and arithmetic as a box stepped alone.
 #include "internal.hpp"
 
 #include <string>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 namespace mhit {
@@ -26,6 +32,8 @@ using namespace mhit;
 
 struct EnsembleGraph {
     hipGraphExec_t exec = nullptr;
+    char* arena = nullptr;              // argument tables of the batched launches
+    size_t arena_cap = 0;
     struct Box {
         double* in[mhit_ctx::kPingPong];    // roles of the ping-pong buffers before the step
         double* out[mhit_ctx::kPingPong];   // ... and after it
@@ -42,11 +50,16 @@ struct mhit_ensemble {
     std::vector<hipEvent_t> join;
     EnsembleGraph graphs[4];
     bool use_graph = true;
+    bool batched = true;                    // one launch per pass and grid size
+    std::vector<hipStream_t> size_stream;   // graph branch of each grid size
+    std::vector<hipEvent_t> size_join;
 };
 
+// The caller has synchronised the ensemble stream (no replay in flight).
 static void clear_ensemble_graphs(mhit_ensemble* e) {
     for (auto& gr : e->graphs) {
         if (gr.exec) (void)hipGraphExecDestroy(gr.exec);
+        if (gr.arena) (void)hipFree(gr.arena);
         gr = EnsembleGraph{};
     }
 }
@@ -99,6 +112,113 @@ static int capture_ensemble_step(mhit_ensemble* e, EnsembleGraph* slot) {
     return MHIT_OK;
 }
 
+// Boxes in order of first appearance of key(box), grouped.
+template <typename K>
+static std::vector<std::vector<mhit_ctx*>> group_by(const std::vector<mhit_ctx*>& boxes, K key) {
+    std::vector<std::vector<mhit_ctx*>> groups;
+    for (mhit_ctx* c : boxes) {
+        bool placed = false;
+        for (auto& gp : groups)
+            if (key(gp[0]) == key(c)) { gp.push_back(c); placed = true; break; }
+        if (!placed) groups.push_back({c});
+    }
+    return groups;
+}
+
+// One timestep of boxes of one grid size on stream st, each pass batched
+// over the boxes that share its kernel and launch shape; the same order of
+// passes as a single box (api.hip, enqueue_step_kernels).
+static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar, hipStream_t st) {
+    int rc;
+    for (mhit_ctx* c : boxes)
+        if (c->P.partflag && c->np > 0) {
+            hipStream_t own = c->stream;
+            c->stream = st;
+            rc = launch_tracers(c);
+            c->stream = own;
+            if (rc) return rc;
+        }
+    std::vector<mhit_ctx*> phase;
+    for (mhit_ctx* c : boxes)
+        if (c->P.phiflag) phase.push_back(c);
+    for (auto& gp : group_by(phase, [](mhit_ctx* c) { return std::make_pair(c->tune.ac_mz, c->tune.xcd); }))
+        if ((rc = launch_acdi_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
+    auto ns_key = [](mhit_ctx* c) {
+        return std::make_tuple(c->P.phiflag != 0, c->fp32, c->tune.ns_mz, c->tune.xcd);
+    };
+    for (auto& gp : group_by(boxes, ns_key))
+        if ((rc = launch_ns_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
+    auto prec_key = [](mhit_ctx* c) { return std::make_pair(c->fp32, c->tune.xcd); };
+    const auto by_prec = group_by(boxes, prec_key);
+    for (auto& gp : by_prec)
+        if ((rc = poisson_project_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
+    for (auto& gp : by_prec)
+        if ((rc = launch_correct_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
+    return launch_finalize_batch(boxes.data(), (int)boxes.size(), ar, st);
+}
+
+static bool can_batch(const mhit_ensemble* e) {
+    if (!e->batched) return false;
+    for (mhit_ctx* c : e->boxes)
+        if (!c->march) return false;
+    return true;
+}
+
+static int capture_ensemble_batched(mhit_ensemble* e, EnsembleGraph* slot) {
+    const size_t nb = e->boxes.size();
+    const auto classes = group_by(e->boxes, [](mhit_ctx* c) { return c->n; });
+    while (e->size_stream.size() < classes.size()) {
+        hipStream_t s = nullptr;
+        hipEvent_t ev = nullptr;
+        MHIT_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        e->size_stream.push_back(s);
+        MHIT_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        e->size_join.push_back(ev);
+    }
+    // at most ~20 tables per size class and a few hundred bytes per box and pass
+    const size_t cap = 8192 * nb + 16384 * classes.size();
+    if (slot->arena_cap < cap) {
+        if (slot->arena) (void)hipFree(slot->arena);
+        slot->arena = nullptr;
+        slot->arena_cap = 0;
+        MHIT_HIP_TRY(hipMalloc((void**)&slot->arena, cap));
+        slot->arena_cap = cap;
+    }
+    ArgArena ar;
+    ar.dev = slot->arena;
+    ar.cap = slot->arena_cap;
+    slot->box.resize(nb);
+    for (size_t b = 0; b < nb; b++) {
+        pingpong_current(e->boxes[b], slot->box[b].in);
+        slot->box[b].tag = e->boxes[b]->ptag;
+        slot->box[b].gen = e->boxes[b]->graph_gen;
+    }
+    hipGraph_t graph = nullptr;
+    MHIT_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t err = hipEventRecord(e->fork, e->stream);
+    int rc = MHIT_OK;
+    for (size_t k = 0; k < classes.size() && err == hipSuccess && rc == MHIT_OK; k++) {
+        hipStream_t st = e->size_stream[k];
+        err = hipStreamWaitEvent(st, e->fork, 0);
+        if (err == hipSuccess) rc = enqueue_size_class(classes[k], ar, st);   // role swaps on the host
+        if (err == hipSuccess && rc == MHIT_OK) err = hipEventRecord(e->size_join[k], st);
+        if (err == hipSuccess && rc == MHIT_OK) err = hipStreamWaitEvent(e->stream, e->size_join[k], 0);
+    }
+    hipError_t end = hipStreamEndCapture(e->stream, &graph);
+    for (size_t b = 0; b < nb; b++) pingpong_undo(e->boxes[b], slot->box[b].in, slot->box[b].out);
+    if (rc != MHIT_OK || err != hipSuccess || end != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != MHIT_OK) return rc;
+        MHIT_HIP_TRY(err != hipSuccess ? err : end);
+    }
+    err = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    MHIT_HIP_TRY(err);
+    // the tables, before the first replay; nothing in flight reads this slot
+    MHIT_HIP_TRY(hipMemcpy(slot->arena, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
+    return MHIT_OK;
+}
+
 static int ensemble_graph_step(mhit_ensemble* e) {
     int rc;
     // eager part on the ensemble stream, in order with the replays
@@ -117,10 +237,12 @@ static int ensemble_graph_step(mhit_ensemble* e) {
         for (auto& gr : e->graphs)
             if (!gr.exec) { slot = &gr; break; }
         if (!slot) {
+            MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
             clear_ensemble_graphs(e);
             slot = &e->graphs[0];
         }
-        if ((rc = capture_ensemble_step(e, slot))) return rc;
+        if ((rc = can_batch(e) ? capture_ensemble_batched(e, slot) : capture_ensemble_step(e, slot)))
+            return rc;
     }
     MHIT_HIP_TRY(hipGraphLaunch(slot->exec, e->stream));
     for (size_t b = 0; b < e->boxes.size(); b++)
@@ -197,10 +319,22 @@ int mhit_ensemble_step(mhit_ensemble* e, int nsteps) {
     return status;
 }
 
+int mhit_ensemble_set_batched(mhit_ensemble* e, int32_t on) {
+    if (!e) { set_error("null ensemble"); return MHIT_ERR_ARG; }
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+    clear_ensemble_graphs(e);
+    e->batched = on != 0;
+    return MHIT_OK;
+}
+
 int mhit_ensemble_destroy(mhit_ensemble* e) {
     if (!e) return MHIT_OK;
     if (e->stream) (void)hipStreamSynchronize(e->stream);
     clear_ensemble_graphs(e);
+    for (hipEvent_t ev : e->size_join)
+        if (ev) (void)hipEventDestroy(ev);
+    for (hipStream_t st : e->size_stream)
+        if (st) (void)hipStreamDestroy(st);
     for (mhit_ctx* c : e->boxes) mhit_destroy(c);
     for (hipEvent_t ev : e->join)
         if (ev) (void)hipEventDestroy(ev);

@@ -200,11 +200,11 @@ __device__ __forceinline__ void fft_last8_from_lds(const V* lds, int c, int t, V
 // twiddles from there: the C threads of one butterfly row share an address
 // (broadcast), and no stage waits on a global load.
 template <typename V, int N, int NT>
-__device__ __forceinline__ void stage_twiddles(V* stw, const V* __restrict__ tw, int tid) {
+__device__ __forceinline__ void stage_twiddles(V* stw, const MHIT_GLOBAL V* __restrict__ tw, int tid) {
 #pragma unroll
     for (int r = 0; r < (N + NT - 1) / NT; r++) {
         const int e = tid + r * NT;
-        if (N % NT == 0 || e < N) stw[e] = tw[e];
+        if (N % NT == 0 || e < N) stw[e] = gload(tw + e);
     }
 }
 // ---------------------------------------------------------------- config

@@ -4,7 +4,9 @@
 
 #include "common.hpp"
 
+#include <cstring>
 #include <utility>
+#include <vector>
 
 struct mhit_ctx {
     mhit_params P;
@@ -133,6 +135,26 @@ inline void pingpong_current(mhit_ctx* c, double* in[]) {
     for (int p = 0; p < mhit_ctx::kPingPong; p++) in[p] = *a[p];
 }
 
+// Argument tables of batched launches (several boxes of one size in one
+// launch).  The device block is allocated before a step is captured, so
+// the tables' device addresses are known to the captured launches; the
+// tables are filled on the host during the capture and copied to the
+// device (upload) before the first replay.
+struct ArgArena {
+    char* dev = nullptr;
+    size_t cap = 0;
+    std::vector<unsigned char> host;
+    template <typename A>
+    const A* put(const A* t, int n) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        const size_t bytes = sizeof(A) * (size_t)n;
+        if (!dev || off + bytes > cap) return nullptr;
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, t, bytes);
+        return reinterpret_cast<const A*>(dev + off);
+    }
+};
+
 // Record the end of stage s on the context stream (profiling only).
 inline void prof_mark(mhit_ctx* c, int s) {
     if (!c->prof || c->prof_n >= 40) return;
@@ -147,6 +169,9 @@ bool fft_size_supported(int n);
 // poisson.hip
 enum class XSrc { Divergence, Real };
 int poisson_project(mhit_ctx* c);                       // u*,v*,w* -> pres
+// the projection of the fused path (div u* in pres -> p in pres) for
+// boxes[0..nb) of one size and precision, batched
+int poisson_project_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
 int poisson_solve_field(mhit_ctx* c, const double* rhs_dev);  // rhs -> pres
 int fft_r2c_dev(mhit_ctx* c, const double* in_dev);     // -> spec, nyq
 int fft_c2r_dev(mhit_ctx* c);                           // spec, nyq -> pres
@@ -157,6 +182,8 @@ int launch_ns(mhit_ctx* c);
 
 int launch_correct(mhit_ctx* c);
 int launch_finalize(mhit_ctx* c);
+int launch_correct_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
+int launch_finalize_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
 int launch_diag(mhit_ctx* c, double* host_out /* 8 values */);
 int correct_block_count(int n);
 int diag_block_count(int n);
@@ -165,6 +192,11 @@ int launch_copy_probe(mhit_ctx* c, const void* src, void* dst, size_t bytes);
 // march.hip
 int launch_acdi_march(mhit_ctx* c);
 int launch_ns_march(mhit_ctx* c);
+// Batched forms: boxes[0..nb) share the size, flags and launch shapes; one
+// launch on stream st, argument table in the arena; the role swaps of every
+// box are done on the host as in the single-box form.
+int launch_acdi_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
+int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
 
 // tracers.hip
 int launch_tracers(mhit_ctx* c);

@@ -19,6 +19,7 @@
 #include "internal.hpp"
 
 #include <type_traits>
+#include <vector>
 
 namespace mhit {
 
@@ -50,17 +51,46 @@ constexpr int NS_MX = 64, NS_MY = 16;  // z chunk: Tuning::ns_mz
 constexpr int NS_W1 = NS_MX + 3, NS_H1 = NS_MY + 3;   // velocity: x, y in [-1, M + 1]
 constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // phi: halo 2
 
+// Arguments of one box.  A launch covers one box (`one`) or, for an
+// ensemble, a table of boxes of the same size stacked along the grid's z.
+template <typename TD>
+struct NsArgs {
+    const double *u, *v, *w, *phi, *ru_o, *rv_o, *rw_o;
+    double *ru_n, *rv_n, *rw_n, *us, *vs, *ws;
+    TD* div;
+    double cdiv;
+    const double *mysin, *mycos;
+    Grid g;
+    const DevScalars* S;
+    int zc, xcd;
+};
+
 template <bool PHI, typename TD>
 __global__ __launch_bounds__(NS_MX * NS_MY)
-void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
-                const double* __restrict__ w, const double* __restrict__ phi,
-                const double* __restrict__ ru_o, const double* __restrict__ rv_o,
-                const double* __restrict__ rw_o, double* __restrict__ ru_n,
-                double* __restrict__ rv_n, double* __restrict__ rw_n,
-                double* __restrict__ us, double* __restrict__ vs, double* __restrict__ ws,
-                TD* __restrict__ div, double cdiv,
-                const double* __restrict__ mysin, const double* __restrict__ mycos,
-                Grid g, const DevScalars* __restrict__ S, int zc, int xcd) {
+void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
+    const Blk blk = launch_blk(nbox);
+    NsArgs<TD> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL double* __restrict__ u = gmem(A.u);
+    const MHIT_GLOBAL double* __restrict__ v = gmem(A.v);
+    const MHIT_GLOBAL double* __restrict__ w = gmem(A.w);
+    const MHIT_GLOBAL double* __restrict__ phi = gmem(A.phi);
+    const MHIT_GLOBAL double* __restrict__ ru_o = gmem(A.ru_o);
+    const MHIT_GLOBAL double* __restrict__ rv_o = gmem(A.rv_o);
+    const MHIT_GLOBAL double* __restrict__ rw_o = gmem(A.rw_o);
+    MHIT_GLOBAL double* __restrict__ ru_n = gmem(A.ru_n);
+    MHIT_GLOBAL double* __restrict__ rv_n = gmem(A.rv_n);
+    MHIT_GLOBAL double* __restrict__ rw_n = gmem(A.rw_n);
+    MHIT_GLOBAL double* __restrict__ us = gmem(A.us);
+    MHIT_GLOBAL double* __restrict__ vs = gmem(A.vs);
+    MHIT_GLOBAL double* __restrict__ ws = gmem(A.ws);
+    MHIT_GLOBAL TD* __restrict__ div = gmem(A.div);
+    const double cdiv = A.cdiv;
+    const MHIT_GLOBAL double* __restrict__ mysin = gmem(A.mysin);
+    const MHIT_GLOBAL double* __restrict__ mycos = gmem(A.mycos);
+    const Grid g = A.g;
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    const int zc = A.zc, xcd = A.xcd;
     constexpr int MX = NS_MX, MY = NS_MY, W1 = NS_W1, H1 = NS_H1, W2 = NS_W2, H2 = NS_H2;
     constexpr int NT = MX * MY;
     constexpr int FXW = MX + 2;                          // fx at x = -1..MX
@@ -75,7 +105,7 @@ void k_ns_march(const double* __restrict__ u, const double* __restrict__ v,
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
     int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz, xcd);
+    xcd_tile(tbx, tby, tbz, blk, xcd);
     const int i0 = tbx * MX, j0 = tby * MY;
     const int z0 = tbz * zc;
     const int z1 = min(z0 + zc, n);
@@ -305,11 +335,29 @@ constexpr int AC_MX = 64, AC_MY = 16;  // z chunk: Tuning::ac_mz
 constexpr int AC_W1 = AC_MX + 2, AC_H1 = AC_MY + 2;   // phi ring: halo 1
 constexpr int AC_W2 = AC_MX + 4, AC_H2 = AC_MY + 4;   // psi ring: halo 2
 
+struct AcdiArgs {
+    double* phi_out;
+    const double *phi, *u, *v, *w;
+    double* rphi_o;
+    Grid g;
+    const DevScalars* S;
+    int zc, xcd;
+};
+
 __global__ __launch_bounds__(AC_MX * AC_MY)
-void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
-                  const double* __restrict__ u, const double* __restrict__ v,
-                  const double* __restrict__ w, double* __restrict__ rphi_o,
-                  Grid g, const DevScalars* __restrict__ S, int zc, int xcd) {
+void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
+    const Blk blk = launch_blk(nbox);
+    AcdiArgs A = one;
+    if (tab) A = tab[blk.box];
+    MHIT_GLOBAL double* __restrict__ phi_out = gmem(A.phi_out);
+    const MHIT_GLOBAL double* __restrict__ phi = gmem(A.phi);
+    const MHIT_GLOBAL double* __restrict__ u = gmem(A.u);
+    const MHIT_GLOBAL double* __restrict__ v = gmem(A.v);
+    const MHIT_GLOBAL double* __restrict__ w = gmem(A.w);
+    MHIT_GLOBAL double* __restrict__ rphi_o = gmem(A.rphi_o);
+    const Grid g = A.g;
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    const int zc = A.zc, xcd = A.xcd;
     constexpr int MX = AC_MX, MY = AC_MY, W1 = AC_W1, H1 = AC_H1, W2 = AC_W2, H2 = AC_H2;
     constexpr int NT = MX * MY;
     __shared__ double spsi[3][H2 * W2];
@@ -320,7 +368,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
     int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz, xcd);
+    xcd_tile(tbx, tby, tbz, blk, xcd);
     const int i0 = tbx * MX, j0 = tby * MY;
     const int z0 = tbz * zc;
     const int z1 = min(z0 + zc, n);
@@ -351,7 +399,7 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
     }
     double pf[NPL];
     auto fetch = [&](int k) {
-        const double* __restrict__ pl = phi + (size_t)wrapn(k, n) * plane;
+        const MHIT_GLOBAL double* __restrict__ pl = phi + (size_t)wrapn(k, n) * plane;
 #pragma unroll
         for (int t = 0; t < NPL; t++)
             if ((H2 * W2) % NT == 0 || tid + t * NT < H2 * W2) pf[t] = pl[poff[t]];
@@ -482,47 +530,126 @@ void k_acdi_march(double* __restrict__ phi_out, const double* __restrict__ phi,
 }
 
 // ------------------------------------------------------------- launchers
-int launch_ns_march(mhit_ctx* c) {
-    const int n = c->n;
-    const int zc = n < c->tune.ns_mz ? n : c->tune.ns_mz;
-    dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, zc)), block(NS_MX, NS_MY, 1);
-    const double cdiv = c->g.rho * c->g.dxi / c->g.dt;
-    auto go = [&](auto phi_tag, auto* div) {
-        constexpr bool PH = decltype(phi_tag)::value;
-        using TD = std::remove_pointer_t<decltype(div)>;
-        k_ns_march<PH, TD><<<grid, block, 0, c->stream>>>(
-            c->u, c->v, c->w, PH ? c->phi : nullptr, c->rhsu_o, c->rhsv_o, c->rhsw_o, c->rhsu_n,
-            c->rhsv_n, c->rhsw_n, c->us, c->vs, c->ws, div, cdiv, c->mysin, c->mycos, c->g, c->S, zc,
-            c->tune.xcd);
-    };
-    // the divergence goes to the pressure buffer, which the x R2C pass reads
-    // and the x C2R pass overwrites with p
-    if (c->P.phiflag) {
-        if (c->fp32) go(std::true_type{}, (float*)c->pres);
-        else go(std::true_type{}, (double*)c->pres);
-    } else {
-        if (c->fp32) go(std::false_type{}, (float*)c->pres);
-        else go(std::false_type{}, (double*)c->pres);
-    }
-    prof_mark(c, ST_NS);
-    MHIT_HIP_TRY(hipGetLastError());
-    // the new AB2 history is in the other buffers (neighbours' old values
-    // are read during the pass)
+template <typename TD>
+static NsArgs<TD> ns_args(mhit_ctx* c, TD* div) {
+    NsArgs<TD> a;
+    a.u = c->u; a.v = c->v; a.w = c->w;
+    a.phi = c->P.phiflag ? c->phi : nullptr;
+    a.ru_o = c->rhsu_o; a.rv_o = c->rhsv_o; a.rw_o = c->rhsw_o;
+    a.ru_n = c->rhsu_n; a.rv_n = c->rhsv_n; a.rw_n = c->rhsw_n;
+    a.us = c->us; a.vs = c->vs; a.ws = c->ws;
+    a.div = div;
+    a.cdiv = c->g.rho * c->g.dxi / c->g.dt;
+    a.mysin = c->mysin; a.mycos = c->mycos;
+    a.g = c->g;
+    a.S = c->S;
+    a.zc = c->n < c->tune.ns_mz ? c->n : c->tune.ns_mz;
+    a.xcd = c->tune.xcd;
+    return a;
+}
+
+static AcdiArgs acdi_args(mhit_ctx* c) {
+    AcdiArgs a;
+    a.phi_out = c->phi_next;
+    a.phi = c->phi;
+    a.u = c->u; a.v = c->v; a.w = c->w;
+    a.rphi_o = c->rhsphi_o;
+    a.g = c->g;
+    a.S = c->S;
+    a.zc = c->n < c->tune.ac_mz ? c->n : c->tune.ac_mz;
+    a.xcd = c->tune.xcd;
+    return a;
+}
+
+// the new AB2 history is in the other buffers (neighbours' old values are
+// read during the pass)
+static void ns_swap(mhit_ctx* c) {
     std::swap(c->rhsu_o, c->rhsu_n);
     std::swap(c->rhsv_o, c->rhsv_n);
     std::swap(c->rhsw_o, c->rhsw_n);
+}
+
+// NS pass over boxes[0..nb) of one size, phase flag, Poisson precision and
+// z chunk, one launch; tab is their argument table (nullptr: one box).
+template <bool PH, typename TD>
+static void go_ns(mhit_ctx* c, const NsArgs<TD>& one, const NsArgs<TD>* tab, int nb, hipStream_t st) {
+    const int n = c->n;
+    dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, one.zc) * nb), block(NS_MX, NS_MY, 1);
+    k_ns_march<PH, TD><<<grid, block, 0, st>>>(one, tab, nb);
+}
+
+// The divergence goes to the pressure buffer, which the x R2C pass reads
+// and the x C2R pass overwrites with p.
+int launch_ns_march(mhit_ctx* c) {
+    if (c->P.phiflag) {
+        if (c->fp32) go_ns<true>(c, ns_args(c, (float*)c->pres), (const NsArgs<float>*)nullptr, 1, c->stream);
+        else go_ns<true>(c, ns_args(c, (double*)c->pres), (const NsArgs<double>*)nullptr, 1, c->stream);
+    } else {
+        if (c->fp32) go_ns<false>(c, ns_args(c, (float*)c->pres), (const NsArgs<float>*)nullptr, 1, c->stream);
+        else go_ns<false>(c, ns_args(c, (double*)c->pres), (const NsArgs<double>*)nullptr, 1, c->stream);
+    }
+    prof_mark(c, ST_NS);
+    MHIT_HIP_TRY(hipGetLastError());
+    ns_swap(c);
     return MHIT_OK;
 }
 
-int launch_acdi_march(mhit_ctx* c) {
+template <bool PH, typename TD>
+static int ns_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    std::vector<NsArgs<TD>> t(nb);
+    for (int b = 0; b < nb; b++) t[b] = ns_args(boxes[b], (TD*)boxes[b]->pres);
+    const NsArgs<TD>* tab = arena.put(t.data(), nb);
+    if (!tab) { set_error("batched launch: argument arena full"); return MHIT_ERR_STATE; }
+    go_ns<PH, TD>(boxes[0], t[0], tab, nb, st);
+    MHIT_HIP_TRY(hipGetLastError());
+    for (int b = 0; b < nb; b++) ns_swap(boxes[b]);
+    return MHIT_OK;
+}
+
+int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    mhit_ctx* c = boxes[0];
+    for (int b = 1; b < nb; b++)
+        if (boxes[b]->n != c->n || boxes[b]->P.phiflag != c->P.phiflag || boxes[b]->fp32 != c->fp32 ||
+            boxes[b]->tune.ns_mz != c->tune.ns_mz || boxes[b]->tune.xcd != c->tune.xcd) {
+            set_error("batched NS pass: boxes differ in size, phase flag, precision or launch shape");
+            return MHIT_ERR_ARG;
+        }
+    if (c->P.phiflag) return c->fp32 ? ns_batch<true, float>(boxes, nb, arena, st)
+                                     : ns_batch<true, double>(boxes, nb, arena, st);
+    return c->fp32 ? ns_batch<false, float>(boxes, nb, arena, st)
+                   : ns_batch<false, double>(boxes, nb, arena, st);
+}
+
+static void go_acdi(mhit_ctx* c, const AcdiArgs& one, const AcdiArgs* tab, int nb, hipStream_t st) {
     const int n = c->n;
-    const int zc = n < c->tune.ac_mz ? n : c->tune.ac_mz;
-    dim3 grid(div_up(n, AC_MX), div_up(n, AC_MY), div_up(n, zc)), block(AC_MX, AC_MY, 1);
-    k_acdi_march<<<grid, block, 0, c->stream>>>(c->phi_next, c->phi, c->u, c->v, c->w, c->rhsphi_o,
-                                                c->g, c->S, zc, c->tune.xcd);
+    dim3 grid(div_up(n, AC_MX), div_up(n, AC_MY), div_up(n, one.zc) * nb), block(AC_MX, AC_MY, 1);
+    k_acdi_march<<<grid, block, 0, st>>>(one, tab, nb);
+}
+
+int launch_acdi_march(mhit_ctx* c) {
+    go_acdi(c, acdi_args(c), nullptr, 1, c->stream);
     prof_mark(c, ST_ACDI);
     MHIT_HIP_TRY(hipGetLastError());
     std::swap(c->phi, c->phi_next);
+    return MHIT_OK;
+}
+
+int launch_acdi_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    mhit_ctx* c = boxes[0];
+    std::vector<AcdiArgs> t(nb);
+    for (int b = 0; b < nb; b++) {
+        if (boxes[b]->n != c->n || !boxes[b]->P.phiflag || boxes[b]->tune.ac_mz != c->tune.ac_mz ||
+            boxes[b]->tune.xcd != c->tune.xcd) {
+            set_error("batched ACDI pass: boxes differ in size or launch shape, or have no phase field");
+            return MHIT_ERR_ARG;
+        }
+        t[b] = acdi_args(boxes[b]);
+    }
+    const AcdiArgs* tab = arena.put(t.data(), nb);
+    if (!tab) { set_error("batched launch: argument arena full"); return MHIT_ERR_STATE; }
+    go_acdi(c, t[0], tab, nb, st);
+    MHIT_HIP_TRY(hipGetLastError());
+    for (int b = 0; b < nb; b++) std::swap(boxes[b]->phi, boxes[b]->phi_next);
     return MHIT_OK;
 }
 

@@ -22,6 +22,8 @@
 #include "fft.hpp"
 #include "internal.hpp"
 
+#include <vector>
+
 namespace mhit {
 
 // --------------------------------------------------------------- x forward
@@ -31,22 +33,43 @@ namespace mhit {
 // Input IN: X_IN_FIELD an FP64 field (us), X_IN_DIV the divergence of
 // (us, vs, ws), X_IN_READY a field already in the transform precision
 // (din: the divergence written by the fused NS pass).
+//
+// Every pass takes the arguments of one box, or a table of boxes of the
+// same size (an ensemble) whose grids are stacked along z.
 enum { X_IN_FIELD = 0, X_IN_DIV = 1, X_IN_READY = 2 };
+template <typename T>
+struct XFwdArgs {
+    const double *us, *vs, *ws;
+    const T* din;
+    double cdiv;
+    cplx_t<T> *spec, *nyq;
+    const cplx_t<T>* tw;
+    const DevScalars* S;
+};
 template <typename T, int N, int IN>
 __global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
-void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
-                 const double* __restrict__ ws, const T* __restrict__ din, double cdiv,
-                 cplx_t<T>* __restrict__ spec, cplx_t<T>* __restrict__ nyq,
-                 const cplx_t<T>* __restrict__ tw, const DevScalars* __restrict__ S) {
+void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
     using Cfg = FftCfg<T, N>;
     constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
     __shared__ V stw[N];
+    const Blk blk = launch_blk(nbox);
+    XFwdArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL double* __restrict__ us = gmem(A.us);
+    const MHIT_GLOBAL double* __restrict__ vs = gmem(A.vs);
+    const MHIT_GLOBAL double* __restrict__ ws = gmem(A.ws);
+    const MHIT_GLOBAL T* __restrict__ din = gmem(A.din);
+    const double cdiv = A.cdiv;
+    MHIT_GLOBAL V* __restrict__ spec = gmem(A.spec);
+    MHIT_GLOBAL V* __restrict__ nyq = gmem(A.nyq);
+    const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     if (S && S->halted) return;
-    const int y0 = blockIdx.x * 2 * P;
-    const int z = blockIdx.y;
+    const int y0 = blk.x * 2 * P;
+    const int z = blk.y;
     const int tid = threadIdx.x;
     const int zp = (z + 1 == N) ? 0 : z + 1;
     stage_twiddles<V, N, NT>(stw, tw, tid);
@@ -104,14 +127,14 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
             out.x = (zk.y + zm.y) * (T)0.5;
             out.y = (zm.x - zk.x) * (T)0.5;
         }
-        spec[((size_t)z * N + y0 + rr) * H + k] = out;
+        gstore(spec + ((size_t)z * N + y0 + rr) * H + k, out);
     }
     for (int rr = tid; rr < 2 * P; rr += NT) {
         const V zh = lds[H * LD + (rr >> 1)];
         V out;
         out.x = (rr & 1) ? zh.y : zh.x;
         out.y = (T)0;
-        nyq[(size_t)z * N + y0 + rr] = out;
+        gstore(nyq + (size_t)z * N + y0 + rr, out);
     }
 }
 
@@ -119,20 +142,33 @@ void k_fft_x_fwd(const double* __restrict__ us, const double* __restrict__ vs,
 // C2R: Z[k] = X_a[k] + i X_b[k] over the Hermitian extension; the imaginary
 // parts of the kx = 0 and kx = n/2 coefficients are ignored (a real
 // transform has none).  Real part -> row ya, imaginary part -> row ya+1.
+template <typename T>
+struct XInvArgs {
+    const cplx_t<T> *spec, *nyq;
+    T* pres;
+    const cplx_t<T>* tw;
+    const DevScalars* S;
+};
 template <typename T, int N>
 __global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
-void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict__ nyq,
-                 T* __restrict__ pres, const cplx_t<T>* __restrict__ tw,
-                 const DevScalars* __restrict__ S) {
+void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
     using Cfg = FftCfg<T, N>;
     constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
     __shared__ V stw[N];
+    const Blk blk = launch_blk(nbox);
+    XInvArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL V* __restrict__ spec = gmem(A.spec);
+    const MHIT_GLOBAL V* __restrict__ nyq = gmem(A.nyq);
+    MHIT_GLOBAL T* __restrict__ pres = gmem(A.pres);
+    const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     if (S && S->halted) return;
-    const int y0 = blockIdx.x * 2 * P;
-    const int z = blockIdx.y;
+    const int y0 = blk.x * 2 * P;
+    const int z = blk.y;
     const int tid = threadIdx.x;
     stage_twiddles<V, N, NT>(stw, tw, tid);
 
@@ -142,12 +178,12 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
         if ((H * P) % NT != 0 && e >= H * P) break;
         const int k = e % H, c = e / H;
         const size_t ra = ((size_t)z * N + y0 + 2 * c) * H;
-        const V A = spec[ra + k];
-        const V B = spec[ra + H + k];
+        const V A = gload(spec + ra + k);
+        const V B = gload(spec + ra + H + k);
         if (k == 0) {
             V z0; z0.x = A.x; z0.y = B.x;
-            const V An = nyq[(size_t)z * N + y0 + 2 * c];
-            const V Bn = nyq[(size_t)z * N + y0 + 2 * c + 1];
+            const V An = gload(nyq + (size_t)z * N + y0 + 2 * c);
+            const V Bn = gload(nyq + (size_t)z * N + y0 + 2 * c + 1);
             V zh; zh.x = An.x; zh.y = Bn.x;
             lds[c] = z0;
             lds[H * LD + c] = zh;
@@ -178,33 +214,50 @@ void k_fft_x_inv(const cplx_t<T>* __restrict__ spec, const cplx_t<T>* __restrict
 // the global access order (a fastest when the axis is contiguous).
 // NYQ marks the Nyquist plane, where the columns are y wavenumbers and
 // kx = n/2; otherwise the columns are kx wavenumbers and blockIdx.y is ky.
+template <typename T>
+struct ColArgs {
+    cplx_t<T>* data;
+    size_t bx, by, sa, sc;
+    const cplx_t<T>* tw;
+    const double* kk;
+    double ntot;
+    const DevScalars* S;
+};
 template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool NYQ>
 __global__ __launch_bounds__((C * FftCfg<T, N>::TPC))
-void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, size_t sc,
-                const cplx_t<T>* __restrict__ tw, const double* __restrict__ kk, double ntot,
-                const DevScalars* __restrict__ S) {
+void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
     constexpr int TPC = FftCfg<T, N>::TPC;
     constexpr int LD = AXIS_CONTIG ? C + 1 : C;
     constexpr int NT = C * TPC;
     __shared__ V lds[N * LD];
     __shared__ V stw[N];
+    const Blk blk = launch_blk(nbox);
+    ColArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    MHIT_GLOBAL V* __restrict__ data = gmem(A.data);
+    const size_t bx = A.bx, by = A.by, sa = A.sa, sc = A.sc;
+    const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
+    const MHIT_GLOBAL double* __restrict__ kk = gmem(A.kk);
+    const double ntot = A.ntot;
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     if (S && S->halted) return;
     const int tid = threadIdx.x;
     // Workgroups dispatch to the 8 XCDs round-robin, each XCD with its own
     // L2.  When a row segment is half a 128-byte line, the two workgroups
     // that share the line are renumbered to land on the same XCD, 8
-    // dispatches apart, so the second reads the line from L2.
-    int bxi = blockIdx.x, byi = blockIdx.y;
-    if (!AXIS_CONTIG && C * (int)sizeof(V) == 64 && gridDim.x % 2 == 0 &&
-        (gridDim.x * gridDim.y) % 16 == 0) {
-        const int L = blockIdx.x + gridDim.x * blockIdx.y;
+    // dispatches apart, so the second reads the line from L2.  (A batched
+    // launch stacks whole box grids of a multiple of 16 workgroups, so the
+    // pairing holds in it too.)
+    int bxi = blk.x, byi = blk.y;
+    if (!AXIS_CONTIG && C * (int)sizeof(V) == 64 && blk.nx % 2 == 0 && (blk.nx * blk.ny) % 16 == 0) {
+        const int L = blk.x + blk.nx * blk.y;
         const int r = L % 16;
         const int t = L - r + (r % 8) * 2 + r / 8;
-        bxi = t % gridDim.x;
-        byi = t / gridDim.x;
+        bxi = t % blk.nx;
+        byi = t / blk.nx;
     }
-    V* __restrict__ g = data + bxi * bx + byi * by;
+    MHIT_GLOBAL V* __restrict__ g = data + bxi * bx + byi * by;
 
     static_assert((N * C) % NT == 0, "column tile must be a multiple of the workgroup");
     // Register-direct form (strided columns, N/8 threads per column, radix-8
@@ -221,7 +274,7 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
         const int c = tid % C, t = tid / C;
         V v[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) v[r] = g[(size_t)(t + r * TPC) * sa + c * sc];
+        for (int r = 0; r < 8; r++) v[r] = gload(g + (size_t)(t + r * TPC) * sa + c * sc);
         stage_twiddles<V, N, NT>(stw, tw, tid);
         __shared__ double skk[MODE == 2 ? N : 1];   // per-axis k^2, staged with the data
         if constexpr (MODE == 2) {
@@ -265,7 +318,7 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             fft_last8_from_lds<V, N, LD, true>(lds, c, t, v, stw);
         }
 #pragma unroll
-        for (int r = 0; r < 8; r++) g[(size_t)(t + r * TPC) * sa + c * sc] = v[r];
+        for (int r = 0; r < 8; r++) gstore(g + (size_t)(t + r * TPC) * sa + c * sc, v[r]);
         return;
     }
     {
@@ -275,7 +328,7 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
             const int e = tid + r * NT;
             int a, c;
             if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
-            buf[r] = g[a * sa + c * sc];
+            buf[r] = gload(g + a * sa + c * sc);
         }
         stage_twiddles<V, N, NT>(stw, tw, tid);
 #pragma unroll
@@ -331,45 +384,83 @@ void k_fft_cols(cplx_t<T>* __restrict__ data, size_t bx, size_t by, size_t sa, s
         const int e = tid + r * NT;
         int a, cc;
         if (AXIS_CONTIG) { a = e % N; cc = e / N; } else { cc = e % C; a = e / C; }
-        g[a * sa + cc * sc] = lds[a * LD + cc];
+        gstore(g + a * sa + cc * sc, lds[a * LD + cc]);
     }
 }
 
 // ---------------------------------------------------------------- drivers
+// The argument of box 0 (`one`) and, for a batched launch (arena given),
+// the device table of all nb boxes.
+template <typename A, typename F>
+static int with_table(mhit_ctx* const* boxes, int nb, ArgArena* arena, F build, A& one, const A*& tab) {
+    one = build(boxes[0]);
+    tab = nullptr;
+    if (!arena) return MHIT_OK;
+    std::vector<A> t(nb);
+    for (int b = 0; b < nb; b++) t[b] = build(boxes[b]);
+    tab = arena->put(t.data(), nb);
+    if (!tab) { set_error("batched launch: argument arena full"); return MHIT_ERR_STATE; }
+    return MHIT_OK;
+}
+
+// Passes over boxes[0..nb) of size N and precision T on stream st; a single
+// box is nb = 1 without an arena.
 template <typename T, int N>
 struct PoissonPasses {
     using V = cplx_t<T>;
     using Cfg = FftCfg<T, N>;
     static constexpr int H = N / 2;
+    using Boxes = mhit_ctx* const*;
 
-    static void x_fwd(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
-        dim3 grid(N / (2 * Cfg::PX), N), block(Cfg::PX * Cfg::TPC);
-        const double cdiv = c->g.rho * c->g.dxi / c->g.dt;
-        if (in == X_IN_READY)
-            k_fft_x_fwd<T, N, X_IN_READY><<<grid, block, 0, c->stream>>>(
-                nullptr, nullptr, nullptr, (const T*)c->pres, 0.0, (V*)c->spec, (V*)c->nyq,
-                (const V*)c->tw, c->S);
-        else if (in == X_IN_DIV)
-            k_fft_x_fwd<T, N, X_IN_DIV><<<grid, block, 0, c->stream>>>(
-                a, b, w, nullptr, cdiv, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
-        else
-            k_fft_x_fwd<T, N, X_IN_FIELD><<<grid, block, 0, c->stream>>>(
-                a, nullptr, nullptr, nullptr, 0.0, (V*)c->spec, (V*)c->nyq, (const V*)c->tw, c->S);
+    static int x_fwd(Boxes bx, int nb, ArgArena* ar, hipStream_t st, const double* a, const double* b,
+                     const double* w, int in) {
+        XFwdArgs<T> one;
+        const XFwdArgs<T>* tab;
+        int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            XFwdArgs<T> x;
+            x.us = in == X_IN_READY ? nullptr : a;
+            x.vs = in == X_IN_DIV ? b : nullptr;
+            x.ws = in == X_IN_DIV ? w : nullptr;
+            x.din = in == X_IN_READY ? (const T*)c->pres : nullptr;
+            x.cdiv = in == X_IN_DIV ? c->g.rho * c->g.dxi / c->g.dt : 0.0;
+            x.spec = (V*)c->spec;
+            x.nyq = (V*)c->nyq;
+            x.tw = (const V*)c->tw;
+            x.S = c->S;
+            return x;
+        }, one, tab);
+        if (rc) return rc;
+        dim3 grid(N / (2 * Cfg::PX), N, nb), block(Cfg::PX * Cfg::TPC);
+        if (in == X_IN_READY) k_fft_x_fwd<T, N, X_IN_READY><<<grid, block, 0, st>>>(one, tab, nb);
+        else if (in == X_IN_DIV) k_fft_x_fwd<T, N, X_IN_DIV><<<grid, block, 0, st>>>(one, tab, nb);
+        else k_fft_x_fwd<T, N, X_IN_FIELD><<<grid, block, 0, st>>>(one, tab, nb);
+        return MHIT_OK;
     }
     template <int MODE>
-    static void y_pass(mhit_ctx* c) {
+    static int y_pass(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
         constexpr int CY = Cfg::CY, CN = Cfg::CN;
+        ColArgs<T> one;
+        const ColArgs<T>* tab;
         // spec: columns kx at fixed z; axis y (stride H)
+        int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            return ColArgs<T>{(V*)c->spec, (size_t)CY, (size_t)N * H, (size_t)H, 1,
+                              (const V*)c->tw, c->kk, 0.0, c->S};
+        }, one, tab);
+        if (rc) return rc;
         k_fft_cols<T, N, CY, false, MODE, false>
-            <<<dim3(H / CY, N), dim3(CY * Cfg::TPC), 0, c->stream>>>(
-                (V*)c->spec, CY, (size_t)N * H, H, 1, (const V*)c->tw, c->kk, 0.0, c->S);
+            <<<dim3(H / CY, N, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
         // nyq [z][y]: columns z, axis y contiguous
+        rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            return ColArgs<T>{(V*)c->nyq, (size_t)CN * N, 0, 1, (size_t)N,
+                              (const V*)c->tw, c->kk, 0.0, c->S};
+        }, one, tab);
+        if (rc) return rc;
         k_fft_cols<T, N, CN, true, MODE, true>
-            <<<dim3(N / CN, 1), dim3(CN * Cfg::TPC), 0, c->stream>>>(
-                (V*)c->nyq, (size_t)CN * N, 0, 1, N, (const V*)c->tw, c->kk, 0.0, c->S);
+            <<<dim3(N / CN, 1, nb), dim3(CN * Cfg::TPC), 0, st>>>(one, tab, nb);
+        return MHIT_OK;
     }
     template <int MODE>
-    static void z_pass(mhit_ctx* c) {
+    static int z_pass(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
         // FP32: half the y pass's columns per workgroup (64-byte row
         // segments): the two FFTs of the z pass make it the longest-running
         // column kernel, and smaller workgroups (two per CU at its register
@@ -378,41 +469,74 @@ struct PoissonPasses {
         // 0.55 ms).
         constexpr int CY = (sizeof(T) == 4) ? Cfg::CY / 2 : Cfg::CY;
         const double ntot = (double)((long long)N * N * N);
+        ColArgs<T> one;
+        const ColArgs<T>* tab;
         // spec: columns kx at fixed y (blockIdx.y); axis z (stride N*H)
+        int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            return ColArgs<T>{(V*)c->spec, (size_t)CY, (size_t)H, (size_t)N * H, 1,
+                              (const V*)c->tw, c->kk, ntot, c->S};
+        }, one, tab);
+        if (rc) return rc;
         k_fft_cols<T, N, CY, false, MODE, false>
-            <<<dim3(H / CY, N), dim3(CY * Cfg::TPC), 0, c->stream>>>(
-                (V*)c->spec, CY, H, (size_t)N * H, 1, (const V*)c->tw, c->kk, ntot, c->S);
+            <<<dim3(H / CY, N, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
         // nyq [z][y]: columns y, axis z (stride N)
+        rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            return ColArgs<T>{(V*)c->nyq, (size_t)CY, 0, (size_t)N, 1,
+                              (const V*)c->tw, c->kk, ntot, c->S};
+        }, one, tab);
+        if (rc) return rc;
         k_fft_cols<T, N, CY, false, MODE, true>
-            <<<dim3(N / CY, 1), dim3(CY * Cfg::TPC), 0, c->stream>>>(
-                (V*)c->nyq, CY, 0, N, 1, (const V*)c->tw, c->kk, ntot, c->S);
+            <<<dim3(N / CY, 1, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
+        return MHIT_OK;
     }
-    static void x_inv(mhit_ctx* c) {
-        dim3 grid(N / (2 * Cfg::PX), N), block(Cfg::PX * Cfg::TPC);
-        k_fft_x_inv<T, N><<<grid, block, 0, c->stream>>>(
-            (const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S);
+    static int x_inv(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
+        XInvArgs<T> one;
+        const XInvArgs<T>* tab;
+        int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+            return XInvArgs<T>{(const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S};
+        }, one, tab);
+        if (rc) return rc;
+        dim3 grid(N / (2 * Cfg::PX), N, nb), block(Cfg::PX * Cfg::TPC);
+        k_fft_x_inv<T, N><<<grid, block, 0, st>>>(one, tab, nb);
+        return MHIT_OK;
     }
-    static void project(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
-        x_fwd(c, a, b, w, in);
+    static int project(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
+        mhit_ctx* const bx[1] = {c};
+        hipStream_t st = c->stream;
+        int rc;
+        if ((rc = x_fwd(bx, 1, nullptr, st, a, b, w, in))) return rc;
         prof_mark(c, ST_FFT_X_FWD);
-        y_pass<0>(c);
+        if ((rc = y_pass<0>(bx, 1, nullptr, st))) return rc;
         prof_mark(c, ST_FFT_Y_FWD);
-        z_pass<2>(c);
+        if ((rc = z_pass<2>(bx, 1, nullptr, st))) return rc;
         prof_mark(c, ST_FFT_Z);
-        y_pass<1>(c);
+        if ((rc = y_pass<1>(bx, 1, nullptr, st))) return rc;
         prof_mark(c, ST_FFT_Y_INV);
-        x_inv(c);
+        if ((rc = x_inv(bx, 1, nullptr, st))) return rc;
         prof_mark(c, ST_FFT_X_INV);
+        return MHIT_OK;
     }
-    static void r2c(mhit_ctx* c, const double* in) {
-        x_fwd(c, in, nullptr, nullptr, X_IN_FIELD);
-        y_pass<0>(c);
-        z_pass<0>(c);
+    static int project_batch(Boxes bx, int nb, ArgArena& ar, hipStream_t st) {
+        int rc;
+        if ((rc = x_fwd(bx, nb, &ar, st, nullptr, nullptr, nullptr, X_IN_READY))) return rc;
+        if ((rc = y_pass<0>(bx, nb, &ar, st))) return rc;
+        if ((rc = z_pass<2>(bx, nb, &ar, st))) return rc;
+        if ((rc = y_pass<1>(bx, nb, &ar, st))) return rc;
+        return x_inv(bx, nb, &ar, st);
     }
-    static void c2r(mhit_ctx* c) {
-        z_pass<1>(c);
-        y_pass<1>(c);
-        x_inv(c);
+    static int r2c(mhit_ctx* c, const double* in) {
+        mhit_ctx* const bx[1] = {c};
+        int rc;
+        if ((rc = x_fwd(bx, 1, nullptr, c->stream, in, nullptr, nullptr, X_IN_FIELD))) return rc;
+        if ((rc = y_pass<0>(bx, 1, nullptr, c->stream))) return rc;
+        return z_pass<0>(bx, 1, nullptr, c->stream);
+    }
+    static int c2r(mhit_ctx* c) {
+        mhit_ctx* const bx[1] = {c};
+        int rc;
+        if ((rc = z_pass<1>(bx, 1, nullptr, c->stream))) return rc;
+        if ((rc = y_pass<1>(bx, 1, nullptr, c->stream))) return rc;
+        return x_inv(bx, 1, nullptr, c->stream);
     }
 };
 
@@ -432,15 +556,16 @@ enum class Op { Project, ProjectReady, Solve, R2C, C2R };
 
 template <typename T>
 static int dispatch(mhit_ctx* c, Op op, const double* a, const double* b, const double* w) {
+    int rc = MHIT_OK;
     switch (c->n) {
-#define CASE(NN)                                                                   \
-    case NN:                                                                       \
-        if (op == Op::Project) PoissonPasses<T, NN>::project(c, a, b, w, X_IN_DIV);  \
-        else if (op == Op::ProjectReady)                                          \
-            PoissonPasses<T, NN>::project(c, a, b, w, X_IN_READY);                \
-        else if (op == Op::Solve) PoissonPasses<T, NN>::project(c, a, b, w, X_IN_FIELD);\
-        else if (op == Op::R2C) PoissonPasses<T, NN>::r2c(c, a);                  \
-        else PoissonPasses<T, NN>::c2r(c);                                         \
+#define CASE(NN)                                                                        \
+    case NN:                                                                            \
+        if (op == Op::Project) rc = PoissonPasses<T, NN>::project(c, a, b, w, X_IN_DIV);  \
+        else if (op == Op::ProjectReady)                                               \
+            rc = PoissonPasses<T, NN>::project(c, a, b, w, X_IN_READY);                \
+        else if (op == Op::Solve) rc = PoissonPasses<T, NN>::project(c, a, b, w, X_IN_FIELD);\
+        else if (op == Op::R2C) rc = PoissonPasses<T, NN>::r2c(c, a);                  \
+        else rc = PoissonPasses<T, NN>::c2r(c);                                         \
         break;
         MHIT_FOR_EACH_N(CASE)
 #undef CASE
@@ -448,6 +573,23 @@ static int dispatch(mhit_ctx* c, Op op, const double* a, const double* b, const 
             set_error("unsupported grid size for the FFT");
             return MHIT_ERR_ARG;
     }
+    if (rc) return rc;
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+template <typename T>
+static int dispatch_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    int rc = MHIT_OK;
+    switch (boxes[0]->n) {
+#define CASE(NN) case NN: rc = PoissonPasses<T, NN>::project_batch(boxes, nb, arena, st); break;
+        MHIT_FOR_EACH_N(CASE)
+#undef CASE
+        default:
+            set_error("unsupported grid size for the FFT");
+            return MHIT_ERR_ARG;
+    }
+    if (rc) return rc;
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
@@ -461,6 +603,15 @@ static int run(mhit_ctx* c, Op op, const double* a, const double* b, const doubl
 int poisson_project(mhit_ctx* c) {
     return c->march ? run(c, Op::ProjectReady, nullptr, nullptr, nullptr)
                     : run(c, Op::Project, c->us, c->vs, c->ws);
+}
+int poisson_project_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    mhit_ctx* c = boxes[0];
+    for (int b = 0; b < nb; b++)
+        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || !boxes[b]->march) {
+            set_error("batched projection: boxes differ in size or precision, or do not use the fused passes");
+            return MHIT_ERR_ARG;
+        }
+    return c->fp32 ? dispatch_batch<float>(boxes, nb, arena, st) : dispatch_batch<double>(boxes, nb, arena, st);
 }
 int poisson_solve_field(mhit_ctx* c, const double* rhs) { return run(c, Op::Solve, rhs, nullptr, nullptr); }
 int fft_r2c_dev(mhit_ctx* c, const double* in) { return run(c, Op::R2C, in, nullptr, nullptr); }

@@ -253,13 +253,34 @@ static dim3 correct_grid(int n) {
     return dim3(div_up(n, CO_MX), div_up(n, CO_MY), div_up(n, zc));
 }
 
+// Arguments of one box (a batched launch reads a table of them).
+template <typename T>
+struct CorrArgs {
+    const double *us, *vs, *ws;
+    const T* p;
+    double *u, *v, *w, *partials;
+    Grid g;
+    const DevScalars* S;
+    int zc, xcd;
+};
+
 template <typename T>
 __global__ __launch_bounds__(CO_MX * CO_MY)
-void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
-               const double* __restrict__ ws, const T* __restrict__ p,
-               double* __restrict__ u, double* __restrict__ v, double* __restrict__ w,
-               double* __restrict__ partials, Grid g, const DevScalars* __restrict__ S,
-               int zc, int xcd) {
+void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
+    const Blk blk = launch_blk(nbox);
+    CorrArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL double* __restrict__ us = gmem(A.us);
+    const MHIT_GLOBAL double* __restrict__ vs = gmem(A.vs);
+    const MHIT_GLOBAL double* __restrict__ ws = gmem(A.ws);
+    const MHIT_GLOBAL T* __restrict__ p = gmem(A.p);
+    MHIT_GLOBAL double* __restrict__ u = gmem(A.u);
+    MHIT_GLOBAL double* __restrict__ v = gmem(A.v);
+    MHIT_GLOBAL double* __restrict__ w = gmem(A.w);
+    MHIT_GLOBAL double* __restrict__ partials = gmem(A.partials);
+    const Grid g = A.g;
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    const int zc = A.zc, xcd = A.xcd;
     constexpr int MX = CO_MX, MY = CO_MY, NT = MX * MY, W = MX + 1, H = MY + 1;
     constexpr int NP = (W * H + NT - 1) / NT;
     __shared__ T sp[W * H];                   // x = -1..MX-1, y = -1..MY-1
@@ -267,7 +288,7 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
     int tbx, tby, tbz;
-    xcd_tile(tbx, tby, tbz, xcd);
+    xcd_tile(tbx, tby, tbz, blk, xcd);
     const int i0 = tbx * MX, j0 = tby * MY, z0 = tbz * zc;
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
@@ -287,7 +308,7 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
         }
         T pf[NP];
         auto fetch = [&](int k) {
-            const T* __restrict__ pl = p + (size_t)k * n * n;
+            const MHIT_GLOBAL T* __restrict__ pl = p + (size_t)k * n * n;
 #pragma unroll
             for (int t = 0; t < NP; t++)
                 if ((W * H) % NT == 0 || tid + t * NT < W * H) pf[t] = pl[off[t]];
@@ -333,26 +354,59 @@ void k_correct(const double* __restrict__ us, const double* __restrict__ vs,
         for (int t = 1; t < NT / 64; t++) r = (f < 3) ? r + red[f][t] : fmax(r, red[f][t]);
         // partials in tile order, so the reduction order does not depend on
         // the workgroup order (xcd_tile)
-        const size_t b = ((size_t)tbz * gridDim.y + tby) * gridDim.x + tbx;
+        const size_t b = ((size_t)tbz * blk.ny + tby) * blk.nx + tbx;
         partials[b * 6 + f] = r;
     }
 }
 
-int launch_correct(mhit_ctx* c) {
-    const int n = c->n;
-    const int zc = correct_zc(n) < n ? correct_zc(n) : n;
+template <typename T>
+static CorrArgs<T> corr_args(mhit_ctx* c) {
+    CorrArgs<T> a;
+    a.us = c->us; a.vs = c->vs; a.ws = c->ws;
+    a.p = (const T*)c->pres;
+    a.u = c->u; a.v = c->v; a.w = c->w;
+    a.partials = c->partials;
+    a.g = c->g;
+    a.S = c->S;
+    a.zc = correct_zc(c->n) < c->n ? correct_zc(c->n) : c->n;
+    a.xcd = c->tune.xcd;
+    return a;
+}
+
+template <typename T>
+static void go_correct(int n, const CorrArgs<T>& one, const CorrArgs<T>* tab, int nb, hipStream_t st) {
     dim3 grid = correct_grid(n), block(CO_MX, CO_MY, 1);
-    if (c->fp32)
-        k_correct<float><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const float*)c->pres,
-                                                       c->u, c->v, c->w, c->partials, c->g, c->S,
-                                                       zc, c->tune.xcd);
-    else
-        k_correct<double><<<grid, block, 0, c->stream>>>(c->us, c->vs, c->ws, (const double*)c->pres,
-                                                        c->u, c->v, c->w, c->partials, c->g, c->S,
-                                                        zc, c->tune.xcd);
+    grid.z *= nb;
+    k_correct<T><<<grid, block, 0, st>>>(one, tab, nb);
+}
+
+int launch_correct(mhit_ctx* c) {
+    if (c->fp32) go_correct<float>(c->n, corr_args<float>(c), nullptr, 1, c->stream);
+    else go_correct<double>(c->n, corr_args<double>(c), nullptr, 1, c->stream);
     prof_mark(c, ST_CORRECT);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
+}
+
+template <typename T>
+static int correct_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    std::vector<CorrArgs<T>> t(nb);
+    for (int b = 0; b < nb; b++) t[b] = corr_args<T>(boxes[b]);
+    const CorrArgs<T>* tab = arena.put(t.data(), nb);
+    if (!tab) { set_error("batched launch: argument arena full"); return MHIT_ERR_STATE; }
+    go_correct<T>(boxes[0]->n, t[0], tab, nb, st);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+int launch_correct_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    mhit_ctx* c = boxes[0];
+    for (int b = 1; b < nb; b++)
+        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || boxes[b]->tune.xcd != c->tune.xcd) {
+            set_error("batched correction: boxes differ in size, precision or tile order");
+            return MHIT_ERR_ARG;
+        }
+    return c->fp32 ? correct_batch<float>(boxes, nb, arena, st) : correct_batch<double>(boxes, nb, arena, st);
 }
 
 int correct_block_count(int n) {
@@ -367,9 +421,22 @@ int diag_block_count(int n) {
 // Finalize: means (main.f90:964-974), signed max after the mean removal
 // (998-1002, max(u - m) = max(u) - m as rounding is monotone), CFL
 // (1003-1006), AB2 switch (801-802).  One block, fixed reduction order.
+struct FinArgs {
+    const double* partials;
+    int nblocks, cfl_abort;
+    Grid g;
+    DevScalars* S;
+};
+
+// One workgroup per box (blockIdx.x is the box of a batched launch).
 __global__ __launch_bounds__(1024)
-void k_finalize(const double* __restrict__ partials, int nblocks, Grid g, int cfl_abort,
-                DevScalars* __restrict__ S) {
+void k_finalize(FinArgs one, const FinArgs* __restrict__ tab) {
+    FinArgs A = one;
+    if (tab) A = tab[blockIdx.x];
+    const MHIT_GLOBAL double* __restrict__ partials = gmem(A.partials);
+    const int nblocks = A.nblocks, cfl_abort = A.cfl_abort;
+    const Grid g = A.g;
+    MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     __shared__ double red[6][1024];
     const int t = threadIdx.x;
     if (S->halted) return;
@@ -405,9 +472,29 @@ void k_finalize(const double* __restrict__ partials, int nblocks, Grid g, int cf
     }
 }
 
+static FinArgs fin_args(mhit_ctx* c) {
+    FinArgs a;
+    a.partials = c->partials;
+    a.nblocks = c->npart_blocks;
+    a.cfl_abort = c->P.cfl_abort;
+    a.g = c->g;
+    a.S = c->S;
+    return a;
+}
+
 int launch_finalize(mhit_ctx* c) {
-    k_finalize<<<1, 1024, 0, c->stream>>>(c->partials, c->npart_blocks, c->g, c->P.cfl_abort, c->S);
+    k_finalize<<<1, 1024, 0, c->stream>>>(fin_args(c), nullptr);
     prof_mark(c, ST_FINALIZE);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+int launch_finalize_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
+    std::vector<FinArgs> t(nb);
+    for (int b = 0; b < nb; b++) t[b] = fin_args(boxes[b]);
+    const FinArgs* tab = arena.put(t.data(), nb);
+    if (!tab) { set_error("batched launch: argument arena full"); return MHIT_ERR_STATE; }
+    k_finalize<<<nb, 1024, 0, st>>>(t[0], tab);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }

@@ -1,8 +1,10 @@
 """An ensemble of boxes stepped together equals each box stepped alone.
 
-Boxes of different sizes and physics (single-phase, multiphase with
-tracers) run concurrently on their own streams; each must reproduce, bit
-for bit, the same box advanced by a standalone Solver."""
+Boxes of different sizes, physics (single-phase, multiphase, tracers) and
+Poisson precisions run together, batched (boxes of one size share every
+launch of a pass, grouped by phase flag and precision) or one graph branch
+per box; each must reproduce, bit for bit, the same box advanced by a
+standalone Solver."""
 import numpy as np
 import pytest
 
@@ -13,12 +15,21 @@ BOXES = [
     dict(nx=24, phiflag=0, partflag=0),
     dict(nx=48, phiflag=1, partflag=0, sigma=0.5),
     dict(nx=16, phiflag=0, partflag=1),
+    # more boxes of the same sizes: batched launches over several boxes,
+    # mixed phase flags and precisions within a size
+    dict(nx=32, phiflag=0, partflag=0),
+    dict(nx=32, phiflag=1, partflag=0, sigma=0.5),
+    dict(nx=32, phiflag=1, partflag=0, sigma=0.5, prec=0),
+    dict(nx=24, phiflag=0, partflag=1),
+    dict(nx=48, phiflag=1, partflag=1, sigma=0.5),
 ]
 
 
 def _params(kw):
     from mhit36_mixedp_amd import default_params
-    return default_params(poisson_precision=1, dt=5e-4, mu=0.006, **kw)
+    kw = dict(kw)
+    prec = kw.pop("prec", 1)
+    return default_params(poisson_precision=prec, dt=5e-4, mu=0.006, **kw)
 
 
 def _init(s, seed, kw):
@@ -29,16 +40,21 @@ def _init(s, seed, kw):
         s.init_tracers_uniform(1500, seed)
 
 
-def test_ensemble_equals_solo():
+@pytest.mark.parametrize("batched", [True, False])
+def test_ensemble_equals_solo(batched):
     from mhit36_mixedp_amd import Ensemble, Solver
-    print("[ensemble] 4 boxes 32/24/48/16", flush=True)
+    print(f"[ensemble] {len(BOXES)} boxes of 16..48, batched={batched}", flush=True)
     params = [_params(kw) for kw in BOXES]
-    with Ensemble(params) as ens:
+    with Ensemble(params, batched=batched) as ens:
         solos = [Solver(_params(kw)) for kw in BOXES]
         for i, kw in enumerate(BOXES):
             _init(ens.boxes[i], 11 + i, kw)
             _init(solos[i], 11 + i, kw)
         for chunk in (1, 3, 4):
+            if chunk == 3:
+                # a launch shape of its own for one box of size 32: its
+                # NS pass leaves the shared launch (results unchanged)
+                ens.boxes[4].set_tuning(ns_mz=16)
             ens.step(chunk)
             for s in solos:
                 s.step(chunk)
@@ -56,14 +72,16 @@ def test_ensemble_equals_solo():
             s.close()
 
 
-def test_ensemble_cfl_stop_isolated():
-    """A box stopped on cou > 7 stays frozen; the other boxes go on."""
+@pytest.mark.parametrize("batched", [True, False])
+def test_ensemble_cfl_stop_isolated(batched):
+    """A box stopped on cou > 7 stays frozen; the other boxes go on (in the
+    batched form both boxes share every launch)."""
     from mhit36_mixedp_amd import CFLStop, Ensemble
-    print("[ensemble] CFL isolation", flush=True)
+    print(f"[ensemble] CFL isolation, batched={batched}", flush=True)
     good = _params(dict(nx=16, phiflag=0, partflag=0))
     bad = _params(dict(nx=16, phiflag=0, partflag=0))
     bad.dt = 0.5
-    with Ensemble([good, bad]) as ens:
+    with Ensemble([good, bad], batched=batched) as ens:
         ens.boxes[0].init_taylor_green()
         ens.boxes[1].init_random_velocity(5, kmax=3.0, urms=20.0)
         with pytest.raises(CFLStop):
