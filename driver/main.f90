@@ -159,6 +159,7 @@ program mhit36
   write(*,'(a,es16.8,a,es16.8,a,es12.4)') ' KE ', d%ke, '  dissipation ', d%dissipation, &
       '  max|div u| ', d%maxdiv
   if (p%phiflag == 1) write(*,'(a,es20.12)') ' phi mass ', d%phi_mass
+  call check(mhit_dump_wait(ctx), 'mhit_dump_wait')
   call check(mhit_destroy(ctx), 'mhit_destroy')
 
 contains
@@ -170,20 +171,23 @@ contains
     if (verbose > 0) mhit_diag_compute_light = mhit_diag_compute(ctx, d)
   end function
 
+  ! The field files are written by the library's host thread while the
+  ! following steps run (the previous dump is finished first); tracers,
+  ! drops and the AB2 scalars are small and written here.
   subroutine dump_fields(tt)
     integer, intent(in) :: tt
-    integer :: ff
-    do ff = FIELD_U, FIELD_P
-      call write_field(ctx, ff, tt, n, buf)
-    end do
-    if (p%phiflag == 1) call write_field(ctx, FIELD_PHI, tt, n, buf)
+    integer(c_int32_t) :: mask
+    mask = ibset(ibset(ibset(ibset(0_c_int32_t, FIELD_U), FIELD_V), FIELD_W), FIELD_P)
+    if (p%phiflag == 1) mask = ibset(mask, FIELD_PHI)
+    if (exact) then
+      mask = ibset(ibset(ibset(mask, FIELD_RHSU_O), FIELD_RHSV_O), FIELD_RHSW_O)
+      if (p%phiflag == 1) mask = ibset(mask, FIELD_RHSPHI_O)
+    end if
+    call check(mhit_dump_async(ctx, './output' // c_null_char, int(tt, c_int32_t), mask), &
+               'mhit_dump_async')
     if (p%partflag == 1) call write_tracers(ctx, tt, exact)
     if (drops .and. p%phiflag == 1) call write_drops(ctx, tt)
     if (.not. exact) return
-    do ff = FIELD_RHSU_O, FIELD_RHSW_O
-      call write_field(ctx, ff, tt, n, buf)
-    end do
-    if (p%phiflag == 1) call write_field(ctx, FIELD_RHSPHI_O, tt, n, buf)
     call check(mhit_diag_compute(ctx, d), 'mhit_diag_compute')
     call write_ab2(tt, d%alpha, d%beta, d%gumax)
   end subroutine

@@ -9,7 +9,7 @@ module mhit_c
 
   integer(c_int), parameter :: MHIT_OK = 0, MHIT_ERR_ARG = 1, MHIT_ERR_HIP = 2, &
                                MHIT_ERR_CFL = 3, MHIT_ERR_NOMEM = 4, MHIT_ERR_STATE = 5, &
-                               MHIT_ERR_NOGPU = 6
+                               MHIT_ERR_NOGPU = 6, MHIT_ERR_IO = 7
   integer(c_int), parameter :: MHIT_POISSON_FP64 = 0, MHIT_POISSON_FP32 = 1
   integer(c_int), parameter :: MHIT_K2_CONTINUOUS = 0, MHIT_K2_MODIFIED = 1
   integer(c_int), parameter :: MHIT_INTERP_REFERENCE = 0, MHIT_INTERP_TRILINEAR = 1
@@ -66,6 +66,18 @@ module mhit_c
       type(c_ptr), value :: ctx
       integer(c_int), value :: fieldn
       real(c_double), intent(out) :: host(*)
+    end function
+    ! writefield (multi/readwrite.f90:38-98) from a host thread; dir is a
+    ! NUL-terminated path, mask has bit fieldn set for each field
+    integer(c_int) function mhit_dump_async(ctx, dir, step, mask) bind(c, name='mhit_dump_async')
+      import :: c_int, c_ptr, c_char, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int32_t), value :: step, mask
+    end function
+    integer(c_int) function mhit_dump_wait(ctx) bind(c, name='mhit_dump_wait')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
     end function
     integer(c_int) function mhit_upload_tracers(ctx, n, x, y, z) bind(c, name='mhit_upload_tracers')
       import :: c_int, c_ptr, c_double, c_int64_t

@@ -38,6 +38,7 @@ extern "C" {
 #define MHIT_ERR_NOMEM       4  /* device allocation failed                 */
 #define MHIT_ERR_STATE       5  /* call not valid in the current state      */
 #define MHIT_ERR_NOGPU       6  /* no HIP device visible                    */
+#define MHIT_ERR_IO          7  /* a file could not be written              */
 
 /* Poisson precision: the "mixedp" branch solves in FP32 (README.md:7-8,
  * main.f90:283-284); the FP64 form is the same algorithm in double.      */
@@ -136,6 +137,20 @@ int mhit_destroy(mhit_ctx* ctx);
  * last projection promoted to FP64.                                        */
 int mhit_upload_field(mhit_ctx* ctx, int fieldn, const double* host);
 int mhit_download_field(mhit_ctx* ctx, int fieldn, double* host);
+
+/* Dump fields to <dir>/<name>_%08d.dat (name u, v, w, p, phi, rhsu_o,
+ * rhsv_o, rhsw_o, rhsphi_o), the files of writefield
+ * (multi/readwrite.f90:38-98): the interior n^3 array in FP64, Fortran
+ * order, with the values mhit_download_field returns.  field_mask has bit
+ * (1 << fieldn) set for every field wanted.  The fields are snapshot on
+ * the device in order with the timesteps already enqueued; a host thread
+ * then writes the files while the caller continues (the reference's time
+ * loop waits for its writes).  A call first waits for the previous dump
+ * and returns that dump's error, if any, without starting a new one.
+ * mhit_dump_wait blocks until the pending dump is on disk and returns its
+ * status (MHIT_ERR_IO when a file could not be written).                 */
+int mhit_dump_async(mhit_ctx* ctx, const char* dir, int32_t step, uint32_t field_mask);
+int mhit_dump_wait(mhit_ctx* ctx);
 
 /* Tracers (particles.f90:11-22): positions in [0, 2pi).  Upload resets the
  * previous-step velocity to zero (the first step is Euler).  Download takes

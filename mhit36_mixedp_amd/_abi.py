@@ -15,6 +15,7 @@ ERR_CFL = 3
 ERR_NOMEM = 4
 ERR_STATE = 5
 ERR_NOGPU = 6
+ERR_IO = 7
 
 POISSON_FP64 = 0
 POISSON_FP32 = 1
@@ -175,6 +176,8 @@ EXPORTS = [
     ("mhit_destroy", ctypes.c_int, [_vp]),
     ("mhit_upload_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     ("mhit_download_field", ctypes.c_int, [_vp, ctypes.c_int, _dp]),
+    ("mhit_dump_async", ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint32]),
+    ("mhit_dump_wait", ctypes.c_int, [_vp]),
     ("mhit_upload_tracers", ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, _dp]),
     ("mhit_upload_tracer_state", ctypes.c_int, [_vp, ctypes.c_int64] + [_dp] * 7),
     ("mhit_download_tracers", ctypes.c_int, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),

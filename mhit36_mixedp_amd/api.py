@@ -103,6 +103,19 @@ class Solver:
         _check(self.lib.mhit_download_field(self._h, fieldn, _ptr(out)), "mhit_download_field")
         return out
 
+    def dump_async(self, directory: str, step: int, fields=(1, 2, 3, 4)):
+        """Write the fields as <directory>/<name>_%08d.dat (the reference's
+        output files) from a host thread; returns at once.  A second call
+        waits for the first; ``dump_wait`` waits for the pending one."""
+        mask = 0
+        for f in fields:
+            mask |= 1 << int(f)
+        _check(self.lib.mhit_dump_async(self._h, str(directory).encode(), int(step), mask),
+               "mhit_dump_async")
+
+    def dump_wait(self):
+        _check(self.lib.mhit_dump_wait(self._h), "mhit_dump_wait")
+
     def set_velocity(self, u, v, w):
         for f, a in zip((_abi.FIELD_U, _abi.FIELD_V, _abi.FIELD_W), (u, v, w)):
             self.upload_field(f, a)

@@ -21,7 +21,7 @@ ARCH = os.environ.get("MHIT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.hip", "poisson.hip", "stencil.hip", "march.hip", "tracers.hip", "init.hip",
            "ensemble.hip", "tune.hip", "drops.hip",
-           "spectra.hip"]
+           "spectra.hip", "dump.hip"]
 
 CXXFLAGS = [
     f"--offload-arch={ARCH}",

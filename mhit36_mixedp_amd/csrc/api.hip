@@ -93,7 +93,7 @@ static void parse_tuning(const char* s, Tuning& t) {
     }
 }
 
-static double* state_field(mhit_ctx* c, int fieldn) {
+double* state_field(mhit_ctx* c, int fieldn) {
     switch (fieldn) {
         case MHIT_FIELD_U: return c->u;
         case MHIT_FIELD_V: return c->v;
@@ -455,6 +455,7 @@ int mhit_destroy(mhit_ctx* c) {
                     c->normz, c->fxst, c->fyst, c->fzst, c->pres, c->spec, c->nyq, c->tw, c->kk,
                     c->mysin, c->partials, c->dpartials, c->S, c->ptag, c->px, c->py, c->pz,
                     c->pu1, c->pv1, c->pw1};
+    dump_release(c);   // waits for a dump still being written
     clear_graphs(c);
     for (void* p : ptrs) dfree(p);
     free_sort_buffers(c);

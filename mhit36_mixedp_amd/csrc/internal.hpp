@@ -8,6 +8,8 @@
 #include <utility>
 #include <vector>
 
+struct mhit_dump_state;   // dump.hip
+
 struct mhit_ctx {
     mhit_params P;
     int n = 0;
@@ -83,6 +85,7 @@ struct mhit_ctx {
     int prof_n = 0;
     double prof_ms[MHIT_PROF_MAX] = {};
     long long prof_calls[MHIT_PROF_MAX] = {};
+    mhit_dump_state* dump = nullptr;   // asynchronous field dumps (mhit_dump_async)
 };
 
 namespace mhit {
@@ -162,6 +165,11 @@ inline void prof_mark(mhit_ctx* c, int s) {
     c->prof_stage[c->prof_n] = s;
     c->prof_n++;
 }
+
+// api.hip: the device array of field id fieldn (1..9 but p), or nullptr
+double* state_field(mhit_ctx* c, int fieldn);
+// dump.hip: wait for a pending dump and free its buffers
+void dump_release(mhit_ctx* c);
 
 // Supported grid sizes (FFT lengths).
 bool fft_size_supported(int n);
