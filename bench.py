@@ -142,7 +142,7 @@ def small_box_comparison(args):
 ENSEMBLE_SIZES = (64, 96, 128)
 
 
-def ensemble_leg(args):
+def ensemble_leg(args, batched_only=False):
     """Ensemble workload: 16 boxes each of 64^3, 96^3 and 128^3 (51,904,512
     cells), every other box multiphase, each with its own seed, viscosity
     and drop count.  Returns box-steps/s stepped together as one captured
@@ -170,6 +170,13 @@ def ensemble_leg(args):
         ens.step(args.ensemble_steps)
         t1 = time.perf_counter()
         together = (t1 - t0) / args.ensemble_steps          # s per ensemble step
+        if batched_only:
+            nbox = len(params)
+            return {"workload": "16x64^3 + 16x96^3 + 16x128^3 boxes, half multiphase, FP32 Poisson",
+                    "boxes": nbox, "cells": cells, "steps": args.ensemble_steps,
+                    "box_steps_per_s": round(nbox / together, 1),
+                    "ms_per_ensemble_step": round(together * 1e3, 4),
+                    "hbm_GBps": round(nbytes / together / 1e9, 1)}
         ens.set_batched(False)
         ens.step(args.warmup)
         t0 = time.perf_counter()
@@ -216,6 +223,8 @@ def main():
     ap.add_argument("--no-copy-probe", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched ensemble and print its JSON (for kernel traces)")
     ap.add_argument("--autotune", action="store_true",
                     help="let mhit_autotune pick the launch shapes before timing (default: the "
                          "library's size-based defaults, which the kernel trace under profiles/ uses)")
@@ -228,6 +237,10 @@ def main():
         sys.exit(2)
 
     from mhit36_mixedp_amd import FIELD_PHI, FIELD_U, FIELD_V, FIELD_W, version
+    if args.ensemble_only:
+        log(f"[bench] {version()}  batched ensemble only")
+        print(json.dumps({"ensemble": ensemble_leg(args, batched_only=True)}), flush=True)
+        return
     log(f"[bench] {version()}  n={args.n} drops={args.ndrops} tracers={args.ntracers}")
 
     # --- FP32-Poisson form (the measured value) --------------------------------
@@ -238,8 +251,12 @@ def main():
         t = s.download_tracers()
         tracers = (t["x"], t["y"], t["z"])
     copy_gbps = s.copy_probe(1 << 30, 10) if not args.no_copy_probe else None
+    # the NS pass's mix: 7 FP64 read streams (u, v, w, phi, 3 rhs_o) and
+    # 6 FP64 write streams (u*, v*, w*, 3 rhs) -- plus div u* in FP32
+    stream_gbps = s.stream_probe(512 << 20, 7, 6, 5) if not args.no_copy_probe else None
     if copy_gbps:
-        log(f"[bench] plain device copy: {copy_gbps:.0f} GB/s (read + write)")
+        log(f"[bench] plain device copy: {copy_gbps:.0f} GB/s (read + write); "
+            f"7-read/6-write streaming: {stream_gbps:.0f} GB/s")
     s_tuning = s.tuning()                     # the shapes timed below
     if args.autotune:
         s_tuning = s.autotune(steps=3)        # set-up, before the timed region
@@ -342,6 +359,8 @@ def main():
             "avg_ms": round(dom_avg_ms, 4),
             "bytes_per_launch": dom["bytes_per_call"],
             "measured_copy_GBps": round(copy_gbps, 1) if copy_gbps else None,
+            "measured_stream_mix_GBps": round(stream_gbps, 1) if stream_gbps else None,
+            "stream_mix": "pointwise kernel with the NS pass's 7 FP64 read + 6 FP64 write streams",
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
         "tuning": s_tuning,

@@ -303,6 +303,13 @@ int mhit_profile_read(mhit_ctx* ctx, mhit_profile* out);
  * (read + write) GB/s.  The measured copy ceiling of the GPU, and the
  * calibration kernel for HBM byte counters.  No reference counterpart.    */
 int mhit_copy_probe(mhit_ctx* ctx, int64_t bytes, int iters, double* gbps);
+/* Pointwise streaming over `reads` FP64 input and `writes` FP64 output
+ * arrays of bytes_per_stream bytes each (1..8 of each), repeated iters
+ * times; *gbps = achieved (read + write) GB/s.  The ceiling for a pass
+ * with that mix of concurrent streams and no stencil.  No reference
+ * counterpart.                                                            */
+int mhit_stream_probe(mhit_ctx* ctx, int64_t bytes_per_stream, int reads, int writes, int iters,
+                      double* gbps);
 
 /* Library version string.                                                 */
 const char* mhit_version(void);

@@ -205,6 +205,7 @@ EXPORTS = [
     ("mhit_profile_enable", ctypes.c_int, [_vp, ctypes.c_int]),
     ("mhit_profile_read", ctypes.c_int, [_vp, _P(Profile)]),
     ("mhit_copy_probe", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, _dp]),
+    ("mhit_stream_probe", ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     ("mhit_ensemble_create", ctypes.c_int, [ctypes.c_int32, _P(Params), _P(_vp)]),
     ("mhit_ensemble_box", ctypes.c_int, [_vp, ctypes.c_int32, _P(_vp)]),
     ("mhit_ensemble_step", ctypes.c_int, [_vp, ctypes.c_int]),

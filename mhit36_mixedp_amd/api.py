@@ -234,6 +234,16 @@ class Solver:
                "mhit_copy_probe")
         return g.value
 
+    def stream_probe(self, nbytes_per_stream: int = 512 << 20, reads: int = 7, writes: int = 6,
+                     iters: int = 5) -> float:
+        """Rate in GB/s (read + write) of a pointwise kernel over `reads`
+        input and `writes` output FP64 arrays: the ceiling for a pass with
+        that stream mix."""
+        g = ctypes.c_double()
+        _check(self.lib.mhit_stream_probe(self._h, int(nbytes_per_stream), int(reads), int(writes),
+                                          int(iters), ctypes.byref(g)), "mhit_stream_probe")
+        return g.value
+
     def profile(self) -> list:
         """Per-stage event timing accumulated since profile_enable()."""
         p = _abi.Profile()

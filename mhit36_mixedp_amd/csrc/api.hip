@@ -633,6 +633,45 @@ int mhit_copy_probe(mhit_ctx* c, int64_t bytes, int iters, double* gbps) {
     return rc;
 }
 
+int mhit_stream_probe(mhit_ctx* c, int64_t bytes_per_stream, int reads, int writes, int iters, double* gbps) {
+    if (!c || reads < 1 || reads > 8 || writes < 1 || writes > 8 || iters < 1 || !gbps ||
+        bytes_per_stream < 8) {
+        set_error("bad arguments");
+        return MHIT_ERR_ARG;
+    }
+    const size_t n = (size_t)bytes_per_stream / sizeof(double);
+    std::vector<double*> in(reads, nullptr), out(writes, nullptr);
+    int rc = MHIT_OK;
+    for (auto* v : {&in, &out})
+        for (double*& p : *v)
+            if (rc == MHIT_OK) rc = dmalloc(&p, n);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (rc == MHIT_OK) {
+        rc = MHIT_ERR_HIP;
+        bool ok = true;
+        for (double* p : in) ok = ok && hipMemsetAsync(p, 0, n * sizeof(double), c->stream) == hipSuccess;
+        if (ok && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess &&
+            launch_stream_probe(c, in.data(), reads, out.data(), writes, n) == MHIT_OK &&   // warm-up
+            hipEventRecord(e0, c->stream) == hipSuccess) {
+            for (int t = 0; t < iters && ok; t++)
+                ok = launch_stream_probe(c, in.data(), reads, out.data(), writes, n) == MHIT_OK;
+            float ms = 0.0f;
+            if (ok && hipEventRecord(e1, c->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+                *gbps = (double)(reads + writes) * n * sizeof(double) * iters / (ms * 1e-3) / 1e9;
+                rc = MHIT_OK;
+            }
+        }
+        if (rc) set_error("stream probe failed");
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(c->stream);
+    for (double* p : in) dfree(p);
+    for (double* p : out) dfree(p);
+    return rc;
+}
+
 int mhit_profile_read(mhit_ctx* c, mhit_profile* out) {
     if (!c || !out) return MHIT_ERR_ARG;
     std::memset(out, 0, sizeof(*out));

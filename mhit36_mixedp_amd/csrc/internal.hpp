@@ -196,6 +196,7 @@ int launch_diag(mhit_ctx* c, double* host_out /* 8 values */);
 int correct_block_count(int n);
 int diag_block_count(int n);
 int launch_copy_probe(mhit_ctx* c, const void* src, void* dst, size_t bytes);
+int launch_stream_probe(mhit_ctx* c, const double* const* in, int R, double* const* out, int W, size_t n);
 
 // march.hip
 int launch_acdi_march(mhit_ctx* c);

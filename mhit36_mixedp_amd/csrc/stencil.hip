@@ -9,6 +9,7 @@
 // which yields the same FP64 values without a second pass over u,v,w.
 #include "internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -586,6 +587,37 @@ void k_copy_probe(const double2* __restrict__ src, double2* __restrict__ dst, si
 int launch_copy_probe(mhit_ctx* c, const void* src, void* dst, size_t bytes) {
     const size_t n2 = bytes / sizeof(double2);
     k_copy_probe<<<div_up((long long)n2, 1024), 256, 0, c->stream>>>((const double2*)src, (double2*)dst, n2);
+    MHIT_HIP_TRY(hipGetLastError());
+    return MHIT_OK;
+}
+
+// Pointwise streaming over R read and W write FP64 arrays (R, W <= 8): the
+// rate a kernel with a given mix of concurrent streams can reach without
+// any stencil, halo or barrier (a practical ceiling for the fused passes).
+struct StreamProbe {
+    const double* in[8];
+    double* out[8];
+};
+
+__global__ __launch_bounds__(256)
+void k_stream_probe(StreamProbe p, int R, int W, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+            if (r < R) s += p.in[r][i];
+#pragma unroll
+        for (int w = 0; w < 8; w++)
+            if (w < W) p.out[w][i] = s + (double)w;
+    }
+}
+
+int launch_stream_probe(mhit_ctx* c, const double* const* in, int R, double* const* out, int W, size_t n) {
+    StreamProbe p{};
+    for (int r = 0; r < R; r++) p.in[r] = in[r];
+    for (int w = 0; w < W; w++) p.out[w] = out[w];
+    const int blocks = (int)std::min<size_t>(32768, (n + 255) / 256);
+    k_stream_probe<<<blocks, 256, 0, c->stream>>>(p, R, W, n);
     MHIT_HIP_TRY(hipGetLastError());
     return MHIT_OK;
 }
