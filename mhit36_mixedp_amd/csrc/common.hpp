@@ -143,15 +143,25 @@ __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, const Blk& b
     bz = L / (nx * ny);
 }
 
-// ln(a/b) for a, b > 0 without rounding the quotient first, in ~45 VALU
-// operations (one division) instead of a division plus a library log.
+// 1/x for a normal x > 0: the hardware estimate (v_rcp_f64) refined by one
+// Newton step, r' = r + r (1 - x r) in two FMAs: within 1 ulp of 1/x, in
+// three operations instead of the ten of an IEEE division.
+__device__ __forceinline__ double rcp_refined(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+
+// ln(a/b) for a, b > 0 without rounding the quotient first, in ~40 VALU
+// operations (one reciprocal) instead of a division plus a library log.
 // Method: a = ma 2^ea, b = mb 2^eb (frexp); move a factor 2 between ma and
 // the exponent so that q = ma/mb lies in [sqrt(1/2), sqrt(2)); then
 // ln(a/b) = (ea - eb) ln 2 + 2 atanh(s), s = (ma - mb)/(ma + mb), with the
 // series 2 atanh(s) = 2 sum_k s^(2k+1)/(2k+1) (Abramowitz & Stegun 4.1.27)
 // truncated after s^23 (|s| <= 3 - 2 sqrt 2, remainder < 1e-19 relative).
 // ma - mb is exact (Sterbenz); ln 2 is split so that (ea - eb) ln2_hi is
-// exact.  Error <= 2 ulp of the exact ln(a/b).  a <= 0 gives -inf (a = 0)
+// exact.  Error <= 2 ulp of the exact ln(a/b) (checked with every FMA
+// emulated exactly and the hardware reciprocal estimate anywhere within
+// 2 ulp of 1/(ma + mb)).  a <= 0 gives -inf (a = 0)
 // or NaN, as log(a/b) would.
 __device__ __forceinline__ double ln_ratio(double a, double b) {
     constexpr double LN2_HI = 0x1.62e42fee00000p-1;   // ln 2 to 32 bits
@@ -161,7 +171,7 @@ __device__ __forceinline__ double ln_ratio(double a, double b) {
     const double mb = frexp(b, &eb);
     if (ma < mb * 0.70710678118654752440) { ma = ma * 2.0; ea -= 1; }
     else if (ma >= mb * 1.41421356237309504880) { ma = ma * 0.5; ea += 1; }
-    const double s = (ma - mb) / (ma + mb);
+    const double s = (ma - mb) * rcp_refined(ma + mb);
     const double t = s * s;
     double p = 1.0 / 23.0;
     p = fma(p, t, 1.0 / 21.0);

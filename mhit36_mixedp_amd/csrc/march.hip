@@ -275,13 +275,16 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
             if (!last) {
                 sfx[ty * FXW + tx + 1] = sig6 * c0 * 0.5 * (P(k, tx + 1, ty) - P(k, tx - 1, ty)) * dxi;
                 sfy[(ty + 1) * MX + tx] = sig6 * c0 * 0.5 * (P(k, tx, ty + 1) - P(k, tx, ty - 1)) * dxi;
-                if (tid < 2 * MY) {                 // x = -1 and x = MX columns
-                    const int b = tid >> 1, a = (tid & 1) ? MX : -1;
-                    sfx[b * FXW + a + 1] = sig6 * chem(k, a, b) * 0.5 * (P(k, a + 1, b) - P(k, a - 1, b)) * dxi;
-                } else if (tid < 2 * MY + 2 * MX) { // y = -1 and y = MY rows
-                    const int e = tid - 2 * MY;
-                    const int a = e >> 1, b = (e & 1) ? MY : -1;
+                // one edge point per thread of waves 0-2 (no wave takes
+                // both branches): y = -1 and y = MY rows on threads
+                // [0, 2 MX), x = -1 and x = MX columns on [2 MX, 2 MX + 2 MY)
+                if (tid < 2 * MX) {
+                    const int a = tid >> 1, b = (tid & 1) ? MY : -1;
                     sfy[(b + 1) * MX + a] = sig6 * chem(k, a, b) * 0.5 * (P(k, a, b + 1) - P(k, a, b - 1)) * dxi;
+                } else if (tid < 2 * MX + 2 * MY) {
+                    const int e = tid - 2 * MX;
+                    const int b = e >> 1, a = (e & 1) ? MX : -1;
+                    sfx[b * FXW + a + 1] = sig6 * chem(k, a, b) * 0.5 * (P(k, a + 1, b) - P(k, a - 1, b)) * dxi;
                 }
             }
             lds_sync();
@@ -432,7 +435,9 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         const double nx = (PS(k, a + 1, b) - PS(k, a - 1, b));
         const double ny = (PS(k, a, b + 1) - PS(k, a, b - 1));
         const double nz = (PS(k + 1, a, b) - PS(k - 1, a, b));
-        const double normag = 1.0 / (sqrt(nx * nx + ny * ny + nz * nz) + enm);
+        // 1/(|grad psi| + 1e-16), main.f90:559-561, with the refined
+        // hardware reciprocal (within 1 ulp)
+        const double normag = rcp_refined(sqrt(nx * nx + ny * ny + nz * nz) + enm);
         const double val = fmin(PH(k, a, b), 1.0);
         const double G = (val + enm) * (1.0 - val + enm) * g_scale;
         fx = G * (nx * normag);
@@ -446,17 +451,20 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         flux(k, tx, ty, fx, fy, fz);
         sFx[bf][ty * (MX + 2) + tx + 1] = fx;
         sFy[bf][(ty + 1) * MX + tx] = fy;
-        if (tid < 2 * MY) {                 // x = -1 and x = MX columns
-            const int b = tid >> 1, a = (tid & 1) ? MX : -1;
-            double gx, gy, gz;
-            flux(k, a, b, gx, gy, gz);
-            sFx[bf][b * (MX + 2) + a + 1] = gx;
-        } else if (tid < 2 * MY + 2 * MX) { // y = -1 and y = MY rows
-            const int e = tid - 2 * MY;
-            const int a = e >> 1, b = (e & 1) ? MY : -1;
+        // one edge point per thread of waves 0-2 (no wave takes both
+        // branches): y = -1 and y = MY rows on threads [0, 2 MX), x = -1
+        // and x = MX columns on [2 MX, 2 MX + 2 MY)
+        if (tid < 2 * MX) {
+            const int a = tid >> 1, b = (tid & 1) ? MY : -1;
             double gx, gy, gz;
             flux(k, a, b, gx, gy, gz);
             sFy[bf][(b + 1) * MX + a] = gy;
+        } else if (tid < 2 * MX + 2 * MY) {
+            const int e = tid - 2 * MX;
+            const int b = e >> 1, a = (e & 1) ? MX : -1;
+            double gx, gy, gz;
+            flux(k, a, b, gx, gy, gz);
+            sFx[bf][b * (MX + 2) + a + 1] = gx;
         }
         return fz;
     };
