@@ -102,16 +102,16 @@ def cpu_baseline(args, fields, tracers):
     return (t1 - t0) * 1e3 / args.cpu_steps, threads
 
 
-def small_box_comparison(args):
-    """The reference's CPU case: 64^3 single-phase NS, FP64 throughout,
-    nu = 0.006, dt = 5e-4, random-phase |k| <= 8 field.  GPU ms/step over 100
-    steps; CPU oracle median ms/step over 10 steps after 5 warm-up steps, on
-    1 thread and on all threads used for the baseline."""
+def box_comparison(args, n, prec, mu, dt, cpu_runs):
+    """Single-phase NS box of size n from a random-phase |k| <= 8 field:
+    GPU ms/step over 100 steps; CPU oracle median ms/step over the timed
+    steps after 5 warm-up steps, for each (threads, timed steps) in
+    cpu_runs."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle
     from mhit36_mixedp_amd import Solver
-    kw = dict(nx=64, poisson_precision=0, phiflag=0, partflag=0, dt=5e-4, mu=0.006)
+    kw = dict(nx=n, poisson_precision=prec, phiflag=0, partflag=0, dt=dt, mu=mu)
     with Solver(**kw) as s:
         s.init_random_velocity(args.seed, kmax=8.0, urms=1.0)
         vel = s.velocity()
@@ -120,22 +120,33 @@ def small_box_comparison(args):
         t0 = time.perf_counter()
         s.step(100)
         gpu_ms = (time.perf_counter() - t0) * 1e3 / 100
-    res = {"workload": "64^3 single-phase NS, FP64 Poisson, nu=0.006, dt=5e-4",
+    res = {"workload": f"{n}^3 single-phase NS, {'FP32' if prec else 'FP64'} Poisson, nu={mu}, dt={dt}",
            "gpu_ms_per_step": round(gpu_ms, 4)}
-    threads_all = min(16, os.cpu_count() or 1)
-    for th in (1, threads_all):
+    for th, steps in cpu_runs:
         oracle.set_threads(th)
         o = oracle.Oracle(**kw)
         for f, a in zip((1, 2, 3), vel):
             o.set_field(f, a)
         o.step(5)
         ts = []
-        for _ in range(10):
+        for _ in range(steps):
             t0 = time.perf_counter()
             o.step(1)
             ts.append((time.perf_counter() - t0) * 1e3)
         o.close()
         res[f"cpu_ms_per_step_{th}_threads"] = round(float(np.median(ts)), 3)
+    return res
+
+
+def small_box_comparison(args):
+    """The reference's CPU case (64^3 single-phase NS, FP64 throughout,
+    nu = 0.006, dt = 5e-4) on 1 thread and on all threads used for the
+    baseline, 10 timed CPU steps each; and the 256^3 single-phase case
+    (FP32 Poisson, nu = 0.002, dt = 2e-4) for a rate comparison, 3 timed
+    CPU steps on all threads."""
+    threads_all = min(16, os.cpu_count() or 1)
+    res = box_comparison(args, 64, 0, 0.006, 5e-4, [(1, 10), (threads_all, 10)])
+    res["c2_256"] = box_comparison(args, 256, 1, 0.002, 2e-4, [(threads_all, 3)])
     return res
 
 
