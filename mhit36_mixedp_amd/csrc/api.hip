@@ -420,23 +420,23 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
         twd[j].x = std::cos(a); twd[j].y = -std::sin(a);
         twf[j].x = (float)twd[j].x; twf[j].y = (float)twd[j].y;
     }
-    if (hipMemcpy(c->mysin, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->kk, kk.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c->tw, c->fp32 ? (const void*)twf.data() : (const void*)twd.data(), n * csz,
+    if (copy_on_stream(c, c->mysin, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        copy_on_stream(c, c->kk, kk.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        copy_on_stream(c, c->tw, c->fp32 ? (const void*)twf.data() : (const void*)twd.data(), n * csz,
                   hipMemcpyHostToDevice) != hipSuccess) {
         set_error("table upload failed");
         return fail(MHIT_ERR_HIP);
     }
     // zero state; AB2 start of main.f90:440-444
     for (double** f : fields)
-        if (hipMemset(*f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
+        if (fill_on_stream(c, *f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
     if (P->phiflag) {
         double** pf[] = {&c->phi, &c->rhsphi, &c->rhsphi_o, &c->psi, &c->tanh_psi, &c->normx,
                          &c->normy, &c->normz, &c->fxst, &c->fyst, &c->fzst};
         for (double** f : pf)
-            if (hipMemset(*f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
+            if (fill_on_stream(c, *f, 0, N3 * sizeof(double)) != hipSuccess) return fail(MHIT_ERR_HIP);
     }
-    if (hipMemset(c->pres, 0, N3 * rsz) != hipSuccess) return fail(MHIT_ERR_HIP);
+    if (fill_on_stream(c, c->pres, 0, N3 * rsz) != hipSuccess) return fail(MHIT_ERR_HIP);
     c->S_host = DevScalars{};
     c->S_host.alpha = 1.0;
     c->S_host.beta = 0.0;
@@ -474,16 +474,16 @@ int mhit_upload_field(mhit_ctx* c, int fieldn, const double* host) {
         if (c->fp32) {
             std::vector<float> tmp(N3);
             for (size_t q = 0; q < N3; q++) tmp[q] = (float)host[q];
-            MHIT_HIP_TRY(hipMemcpy(c->pres, tmp.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
+            MHIT_HIP_TRY(copy_on_stream(c, c->pres, tmp.data(), N3 * sizeof(float), hipMemcpyHostToDevice));
         } else {
-            MHIT_HIP_TRY(hipMemcpy(c->pres, host, N3 * sizeof(double), hipMemcpyHostToDevice));
+            MHIT_HIP_TRY(copy_on_stream(c, c->pres, host, N3 * sizeof(double), hipMemcpyHostToDevice));
         }
         return MHIT_OK;
     }
     double* f = state_field(c, fieldn);
     if (!f) { set_error("field not allocated (phiflag=0?) or bad fieldn"); return MHIT_ERR_ARG; }
     if ((rc = sync_scalars(c))) return rc;
-    MHIT_HIP_TRY(hipMemcpy(f, host, N3 * sizeof(double), hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, f, host, N3 * sizeof(double), hipMemcpyHostToDevice));
     if (fieldn <= MHIT_FIELD_W) {
         c->S_host.mean[fieldn - 1] = 0.0;
         if ((rc = push_scalars(c))) return rc;
@@ -499,16 +499,16 @@ int mhit_download_field(mhit_ctx* c, int fieldn, double* host) {
     if (fieldn == MHIT_FIELD_P) {
         if (c->fp32) {
             std::vector<float> tmp(N3);
-            MHIT_HIP_TRY(hipMemcpy(tmp.data(), c->pres, N3 * sizeof(float), hipMemcpyDeviceToHost));
+            MHIT_HIP_TRY(copy_on_stream(c, tmp.data(), c->pres, N3 * sizeof(float), hipMemcpyDeviceToHost));
             for (size_t q = 0; q < N3; q++) host[q] = (double)tmp[q];
         } else {
-            MHIT_HIP_TRY(hipMemcpy(host, c->pres, N3 * sizeof(double), hipMemcpyDeviceToHost));
+            MHIT_HIP_TRY(copy_on_stream(c, host, c->pres, N3 * sizeof(double), hipMemcpyDeviceToHost));
         }
         return MHIT_OK;
     }
     double* f = state_field(c, fieldn);
     if (!f) { set_error("field not allocated (phiflag=0?) or bad fieldn"); return MHIT_ERR_ARG; }
-    MHIT_HIP_TRY(hipMemcpy(host, f, N3 * sizeof(double), hipMemcpyDeviceToHost));
+    MHIT_HIP_TRY(copy_on_stream(c, host, f, N3 * sizeof(double), hipMemcpyDeviceToHost));
     if (fieldn <= MHIT_FIELD_W) {
         // main.f90:977-981 applied on the way out (the device keeps u + mean)
         const double m = c->S_host.mean[fieldn - 1];
@@ -526,13 +526,13 @@ int mhit_upload_tracers(mhit_ctx* c, int64_t n, const double* x, const double* y
     std::vector<double> tag(n);
     for (int64_t i = 0; i < n; i++) tag[i] = (double)(i + 1);   // particles.f90:46-49
     const size_t b = (size_t)n * sizeof(double);
-    MHIT_HIP_TRY(hipMemcpy(c->ptag, tag.data(), b, hipMemcpyHostToDevice));
-    MHIT_HIP_TRY(hipMemcpy(c->px, x, b, hipMemcpyHostToDevice));
-    MHIT_HIP_TRY(hipMemcpy(c->py, y, b, hipMemcpyHostToDevice));
-    MHIT_HIP_TRY(hipMemcpy(c->pz, z, b, hipMemcpyHostToDevice));
-    MHIT_HIP_TRY(hipMemset(c->pu1, 0, b));
-    MHIT_HIP_TRY(hipMemset(c->pv1, 0, b));
-    MHIT_HIP_TRY(hipMemset(c->pw1, 0, b));
+    MHIT_HIP_TRY(copy_on_stream(c, c->ptag, tag.data(), b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, c->px, x, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, c->py, y, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, c->pz, z, b, hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(fill_on_stream(c, c->pu1, 0, b));
+    MHIT_HIP_TRY(fill_on_stream(c, c->pv1, 0, b));
+    MHIT_HIP_TRY(fill_on_stream(c, c->pw1, 0, b));
     return MHIT_OK;
 }
 
@@ -549,7 +549,7 @@ int mhit_upload_tracer_state(mhit_ctx* c, int64_t n, const double* tag, const do
     const size_t b = (size_t)n * sizeof(double);
     struct { double* d; const double* h; } cols[] = {{c->ptag, tag}, {c->px, x}, {c->py, y}, {c->pz, z},
                                                      {c->pu1, up}, {c->pv1, vp}, {c->pw1, wp}};
-    for (auto& col : cols) MHIT_HIP_TRY(hipMemcpy(col.d, col.h, b, hipMemcpyHostToDevice));
+    for (auto& col : cols) MHIT_HIP_TRY(copy_on_stream(c, col.d, col.h, b, hipMemcpyHostToDevice));
     return MHIT_OK;
 }
 
@@ -562,7 +562,7 @@ int mhit_download_tracers(mhit_ctx* c, double* tag, double* x, double* y, double
     struct { double* h; double* d; } cols[] = {{tag, c->ptag}, {x, c->px}, {y, c->py}, {z, c->pz},
                                                {up, c->pu1}, {vp, c->pv1}, {wp, c->pw1}};
     for (auto& col : cols)
-        if (col.h) MHIT_HIP_TRY(hipMemcpy(col.h, col.d, b, hipMemcpyDeviceToHost));
+        if (col.h) MHIT_HIP_TRY(copy_on_stream(c, col.h, col.d, b, hipMemcpyDeviceToHost));
     return MHIT_OK;
 }
 
@@ -797,8 +797,8 @@ int mhit_init_random_velocity(mhit_ctx* c, uint64_t seed, double kmax, double ur
     int rc;
     if ((rc = dmalloc(&dmodes, modes.size()))) return rc;
     if ((rc = dmalloc(&dE, (size_t)n))) { dfree(dmodes); return rc; }
-    hipError_t e1 = hipMemcpy(dmodes, modes.data(), modes.size() * sizeof(double), hipMemcpyHostToDevice);
-    hipError_t e2 = hipMemcpy(dE, E.data(), n * sizeof(double2), hipMemcpyHostToDevice);
+    hipError_t e1 = copy_on_stream(c, dmodes, modes.data(), modes.size() * sizeof(double), hipMemcpyHostToDevice);
+    hipError_t e2 = copy_on_stream(c, dE, E.data(), n * sizeof(double2), hipMemcpyHostToDevice);
     rc = (e1 == hipSuccess && e2 == hipSuccess) ? launch_init_modes(c, dmodes, nmodes, dE) : MHIT_ERR_HIP;
     if (rc == MHIT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
     dfree(dmodes);
@@ -845,7 +845,7 @@ int mhit_init_drops(mhit_ctx* c, int32_t ndrops, double radius, uint64_t seed) {
     if ((rc = mhit_drop_centers(ndrops, radius, 4.0 * c->g.eps, seed, xyz.data()))) return rc;
     double* dc = nullptr;
     if ((rc = dmalloc(&dc, xyz.size()))) return rc;
-    rc = hipMemcpy(dc, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
+    rc = copy_on_stream(c, dc, xyz.data(), xyz.size() * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
              ? launch_init_drops(c, dc, ndrops, radius) : MHIT_ERR_HIP;
     if (rc == MHIT_OK && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
     dfree(dc);
@@ -868,7 +868,7 @@ int mhit_poisson_solve(mhit_ctx* c, const double* rhs, double* p) {
     double* d = nullptr;
     int rc;
     if ((rc = dmalloc(&d, c->N3))) return rc;
-    if (hipMemcpy(d, rhs, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
+    if (copy_on_stream(c, d, rhs, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
     if (!rc) rc = poisson_solve_field(c, d);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
     dfree(d);
@@ -884,8 +884,8 @@ template <typename V>
 static int download_spectrum(mhit_ctx* c, double* out) {
     const int n = c->n, H = n / 2;
     std::vector<V> spec((size_t)n * n * H), nyq((size_t)n * n);
-    MHIT_HIP_TRY(hipMemcpy(spec.data(), c->spec, spec.size() * sizeof(V), hipMemcpyDeviceToHost));
-    MHIT_HIP_TRY(hipMemcpy(nyq.data(), c->nyq, nyq.size() * sizeof(V), hipMemcpyDeviceToHost));
+    MHIT_HIP_TRY(copy_on_stream(c, spec.data(), c->spec, spec.size() * sizeof(V), hipMemcpyDeviceToHost));
+    MHIT_HIP_TRY(copy_on_stream(c, nyq.data(), c->nyq, nyq.size() * sizeof(V), hipMemcpyDeviceToHost));
     for (size_t r = 0; r < (size_t)n * n; r++) {
         for (int k = 0; k < H; k++) {
             out[2 * (r * (H + 1) + k)] = (double)spec[r * H + k].x;
@@ -909,8 +909,8 @@ static int upload_spectrum(mhit_ctx* c, const double* in) {
         nyq[r].x = in[2 * (r * (H + 1) + H)];
         nyq[r].y = in[2 * (r * (H + 1) + H) + 1];
     }
-    MHIT_HIP_TRY(hipMemcpy(c->spec, spec.data(), spec.size() * sizeof(V), hipMemcpyHostToDevice));
-    MHIT_HIP_TRY(hipMemcpy(c->nyq, nyq.data(), nyq.size() * sizeof(V), hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, c->spec, spec.data(), spec.size() * sizeof(V), hipMemcpyHostToDevice));
+    MHIT_HIP_TRY(copy_on_stream(c, c->nyq, nyq.data(), nyq.size() * sizeof(V), hipMemcpyHostToDevice));
     return MHIT_OK;
 }
 
@@ -923,7 +923,7 @@ int mhit_fft_r2c(mhit_ctx* c, const double* in, double* out) {
     double* d = nullptr;
     int rc;
     if ((rc = dmalloc(&d, c->N3))) return rc;
-    if (hipMemcpy(d, in, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
+    if (copy_on_stream(c, d, in, c->N3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = MHIT_ERR_HIP;
     if (!rc) rc = fft_r2c_dev(c, d);
     if (!rc && hipStreamSynchronize(c->stream) != hipSuccess) rc = MHIT_ERR_HIP;
     dfree(d);

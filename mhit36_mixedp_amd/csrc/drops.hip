@@ -208,9 +208,9 @@ int mhit_drop_stats(mhit_ctx* c, double threshold, int connectivity, int64_t max
     *ndrops = count;
     if (count > 0) {
         std::vector<int> hr(count);
-        DROPS_TRY(hipMemcpy(hr.data(), roots, (size_t)count * sizeof(int), hipMemcpyDeviceToHost));
+        DROPS_TRY(copy_on_stream(c, hr.data(), roots, (size_t)count * sizeof(int), hipMemcpyDeviceToHost));
         std::sort(hr.begin(), hr.end());
-        DROPS_TRY(hipMemcpy(roots, hr.data(), (size_t)count * sizeof(int), hipMemcpyHostToDevice));
+        DROPS_TRY(copy_on_stream(c, roots, hr.data(), (size_t)count * sizeof(int), hipMemcpyHostToDevice));
         DROPS_TRY(hipMalloc((void**)&stats, (size_t)count * kStats * sizeof(double)));
         DROPS_TRY(hipMemsetAsync(stats, 0, (size_t)count * kStats * sizeof(double), c->stream));
         k_cc_sums<<<nb, 256, 0, c->stream>>>(L, roots, count, c->u, c->v, c->w, dtab, c->S, stats, n);

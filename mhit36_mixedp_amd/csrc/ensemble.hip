@@ -214,8 +214,11 @@ static int capture_ensemble_batched(mhit_ensemble* e, EnsembleGraph* slot) {
     err = hipGraphInstantiate(&slot->exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     MHIT_HIP_TRY(err);
-    // the tables, before the first replay; nothing in flight reads this slot
-    MHIT_HIP_TRY(hipMemcpy(slot->arena, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice));
+    // the tables, before the first replay (in order with it on the ensemble
+    // stream, and complete before the host staging goes away)
+    MHIT_HIP_TRY(hipMemcpyAsync(slot->arena, ar.host.data(), ar.host.size(), hipMemcpyHostToDevice,
+                                e->stream));
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
     return MHIT_OK;
 }
 

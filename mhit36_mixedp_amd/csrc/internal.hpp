@@ -158,6 +158,22 @@ struct ArgArena {
     }
 };
 
+// A blocking copy in order with the context's stream.  The stream is
+// non-blocking, so a plain hipMemcpy (legacy stream) would not wait for the
+// kernels queued on it, and an upload from pageable memory may return
+// before its DMA has landed; here both are ordered and complete at return.
+inline hipError_t copy_on_stream(mhit_ctx* c, void* dst, const void* src, size_t bytes,
+                                 hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+inline hipError_t fill_on_stream(mhit_ctx* c, void* dst, int value, size_t bytes) {
+    hipError_t e = hipMemsetAsync(dst, value, bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
 // Record the end of stage s on the context stream (profiling only).
 inline void prof_mark(mhit_ctx* c, int s) {
     if (!c->prof || c->prof_n >= 40) return;
