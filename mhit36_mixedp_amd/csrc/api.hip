@@ -366,7 +366,7 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     const size_t N3 = c->N3;
     // state and AB2 history; the other half of the AB2 ping-pong (fused
     // path) or the intermediates of the split ACDI/CSF kernels (cell path)
-    // only where they are used: at 512^3 multiphase, 27 instead of 37 GB
+    // only where they are used: at 512^3 multiphase, 17 instead of 27 GB
     std::vector<double**> fields = {&c->u, &c->v, &c->w, &c->us, &c->vs, &c->ws,
                                     &c->rhsu_o, &c->rhsv_o, &c->rhsw_o};
     if (c->march) fields.insert(fields.end(), {&c->rhsu_n, &c->rhsv_n, &c->rhsw_n});
