@@ -265,8 +265,9 @@ int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
 /* ---- ensembles of independent boxes ----------------------------------- */
 /* nbox independent MHIT36 boxes (sizes, physics switches and parameters
  * may differ per box) advanced together on one GPU.  The reference runs one
- * box per job; this has no reference counterpart (BASELINE ensemble
- * configuration).  Each box is a full context: initialise, upload and
+ * box per job; this has no reference counterpart (it serves ensembles of
+ * small boxes, e.g. 16 each of 64^3, 96^3 and 128^3).  Each box is a full
+ * context: initialise, upload and
  * download through mhit_ensemble_box().  mhit_ensemble_step advances every
  * box nsteps steps as one captured graph; returns MHIT_ERR_CFL if any box
  * stopped on cou > 7 (that box stays frozen, the others keep stepping).

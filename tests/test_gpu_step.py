@@ -2,8 +2,8 @@
 multi/main.f90:456-1007) on identical seeded inputs.
 
 Bars: FP64 Poisson 1e-12 relative after one step and 1e-10 after many
-(the north star's 1e-10 for the FP64 mode); FP32 Poisson against the
-oracle's FP32 form 1e-6 after one step.  Tracer positions and phi use
+(the FP64 mode's tolerance against the reference's FP64 path); FP32
+Poisson against the oracle's FP32 form 1e-6 after one step.  Tracer positions and phi use
 the same bars."""
 import numpy as np
 import pytest
