@@ -242,7 +242,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs one box on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the time-stepper runs one box on one GPU; --gpus {args.gpus} is not supported "
               "(independent boxes shard trivially across GPUs; see the sharding notes).",
               file=sys.stderr)
         sys.exit(2)

@@ -72,7 +72,7 @@ extern "C" {
  * Run parameters.  The first block is input.inp, in its order
  * (multi/input.inp:1-18, multi/readinput.f90:14-35).  The second block is
  * what the reference fixes at compile time (module.f90:2 nx; main.f90:55-57
- * phiflag/partflag) or this build adds as options.
+ * phiflag/partflag) or that this library adds as options.
  */
 typedef struct mhit_params {
     /* input.inp */

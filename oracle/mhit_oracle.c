@@ -1,7 +1,7 @@
 /*
  * mhit_oracle.c -- CPU FP64 restatement of one MHIT36 timestep.
  *
- * TEST INFRASTRUCTURE (the checker), not product code: only tests/,
+ * TEST INFRASTRUCTURE (the oracle), not product code: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
  *
  * Every loop follows the reference loop it cites (aroccon/MHIT36_mixedp,

@@ -1,7 +1,7 @@
 /*
  * mhit_oracle.h -- CPU FP64 restatement of the MHIT36 timestep.
  *
- * TEST INFRASTRUCTURE ONLY.  This is the checker the GPU library is compared
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the GPU library is compared
  * with; it is never linked into, loaded by, or called from the product
  * library (libmhit36.so).  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py use it.
