@@ -40,7 +40,7 @@ extern "C" {
 #define MHIT_ERR_NOGPU       6  /* no HIP device visible                    */
 #define MHIT_ERR_IO          7  /* a file could not be written              */
 
-/* Poisson precision: the "mixedp" branch solves in FP32 (README.md:7-8,
+/* Poisson precision: the "mixedp" branch solves in FP32 (reference README.md:7-8,
  * main.f90:283-284); the FP64 form is the same algorithm in double.      */
 #define MHIT_POISSON_FP64    0
 #define MHIT_POISSON_FP32    1
