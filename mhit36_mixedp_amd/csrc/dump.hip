@@ -142,17 +142,18 @@ int mhit_dump_async(mhit_ctx* c, const char* dir, int32_t step, uint32_t field_m
     if ((rc = join(d))) return rc;   // the previous dump (and its error, if any)
     if (fields.empty()) return MHIT_OK;
     const size_t N3 = c->N3;
-    if (!d->xfer) {
-        MHIT_HIP_TRY(hipStreamCreateWithFlags(&d->xfer, hipStreamNonBlocking));
-        MHIT_HIP_TRY(hipEventCreateWithFlags(&d->ready, hipEventDisableTiming));
-        for (hipEvent_t& e : d->moved) MHIT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        for (double*& p : d->pinned)
-            if (hipHostMalloc((void**)&p, N3 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-                p = nullptr;
-                set_error("dump: no pinned host memory for the transfer buffers");
-                return MHIT_ERR_NOMEM;
-            }
-    }
+    // transfer resources, created on first use; each one checked on its own
+    // so that a call after a failed set-up completes it
+    if (!d->xfer) MHIT_HIP_TRY(hipStreamCreateWithFlags(&d->xfer, hipStreamNonBlocking));
+    if (!d->ready) MHIT_HIP_TRY(hipEventCreateWithFlags(&d->ready, hipEventDisableTiming));
+    for (hipEvent_t& e : d->moved)
+        if (!e) MHIT_HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (double*& p : d->pinned)
+        if (!p && hipHostMalloc((void**)&p, N3 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+            p = nullptr;
+            set_error("dump: no pinned host memory for the transfer buffers");
+            return MHIT_ERR_NOMEM;
+        }
     std::vector<std::string> paths;
     for (int f : fields) {
         if (!d->stage[f] && hipMalloc((void**)&d->stage[f], N3 * sizeof(double)) != hipSuccess) {

@@ -171,8 +171,12 @@ static int capture_ensemble_batched(mhit_ensemble* e, EnsembleGraph* slot) {
         hipStream_t s = nullptr;
         hipEvent_t ev = nullptr;
         MHIT_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-        e->size_stream.push_back(s);
-        MHIT_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+            (void)hipStreamDestroy(s);
+            set_error("ensemble: event creation failed");
+            return MHIT_ERR_HIP;
+        }
+        e->size_stream.push_back(s);   // the pair is kept together
         e->size_join.push_back(ev);
     }
     // at most ~20 tables per size class and a few hundred bytes per box and pass
