@@ -72,9 +72,17 @@ int take(mhit_ctx* c, Snapshot& s) {
         }
         s.copies.push_back(d);
         s.counts.push_back(cnt);
-        MHIT_HIP_TRY(hipMemcpyAsync(d, *r[k], cnt * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        if (hipMemcpyAsync(d, *r[k], cnt * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+            release(s);
+            set_error("autotune: state snapshot copy failed");
+            return MHIT_ERR_HIP;
+        }
     }
-    MHIT_HIP_TRY(hipStreamSynchronize(c->stream));
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        release(s);
+        set_error("autotune: state snapshot copy failed");
+        return MHIT_ERR_HIP;
+    }
     return MHIT_OK;
 }
 
