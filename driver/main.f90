@@ -164,11 +164,12 @@ program mhit36
 
 contains
 
-  ! The scalars of the step (cou) without the field diagnostics pass.
+  ! The scalars of the step (cou, printed as main.f90:1005 does) without
+  ! the field diagnostics pass.
   integer(c_int) function mhit_diag_compute_light()
     d%cou = 0
     mhit_diag_compute_light = MHIT_OK
-    if (verbose > 0) mhit_diag_compute_light = mhit_diag_compute(ctx, d)
+    if (verbose > 0) mhit_diag_compute_light = mhit_scalars(ctx, d)
   end function
 
   ! The field files are written by the library's host thread while the

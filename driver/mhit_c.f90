@@ -128,6 +128,12 @@ module mhit_c
       type(c_ptr), value :: ctx
       type(mhit_diag), intent(out) :: d
     end function
+    ! the last step's scalars (cou, gumax, ...) without a pass over the fields
+    integer(c_int) function mhit_scalars(ctx, d) bind(c, name='mhit_scalars')
+      import :: c_int, c_ptr, mhit_diag
+      type(c_ptr), value :: ctx
+      type(mhit_diag), intent(out) :: d
+    end function
     integer(c_int) function mhit_init_taylor_green(ctx) bind(c, name='mhit_init_taylor_green')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx

@@ -177,6 +177,11 @@ int mhit_step(mhit_ctx* ctx, int nsteps);
 
 /* Scalars of the last step plus diagnostics computed on the current state. */
 int mhit_diag_compute(mhit_ctx* ctx, mhit_diag* out);
+/* The scalars of the last step only (step, alpha, beta, gumax, cou and the
+ * removed means), read back without a pass over the fields: what the
+ * reference prints every step (main.f90:1003-1005).  The diagnostic
+ * members of *out are zero.                                               */
+int mhit_scalars(mhit_ctx* ctx, mhit_diag* out);
 
 /* Set the AB2 / gamma state, e.g. to resume a run (main.f90:440-444).     */
 int mhit_set_ab2_state(mhit_ctx* ctx, double alpha, double beta, double gumax);

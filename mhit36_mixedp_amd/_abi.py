@@ -184,6 +184,7 @@ EXPORTS = [
     ("mhit_tracer_count", ctypes.c_int64, [_vp]),
     ("mhit_step", ctypes.c_int, [_vp, ctypes.c_int]),
     ("mhit_diag_compute", ctypes.c_int, [_vp, _P(Diag)]),
+    ("mhit_scalars", ctypes.c_int, [_vp, _P(Diag)]),
     ("mhit_set_ab2_state", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("mhit_sync", ctypes.c_int, [_vp]),
     ("mhit_get_tuning", ctypes.c_int, [_vp, _P(Tuning)]),

@@ -183,6 +183,13 @@ class Solver:
         _check(self.lib.mhit_diag_compute(self._h, ctypes.byref(d)), "mhit_diag_compute")
         return d.as_dict()
 
+    def scalars(self) -> dict:
+        """The last step's scalars (step, alpha, beta, gumax, cou, means)
+        without a pass over the fields."""
+        d = Diag()
+        _check(self.lib.mhit_scalars(self._h, ctypes.byref(d)), "mhit_scalars")
+        return d.as_dict()
+
     def tuning(self) -> dict:
         t = _abi.Tuning()
         _check(self.lib.mhit_get_tuning(self._h, ctypes.byref(t)), "mhit_get_tuning")

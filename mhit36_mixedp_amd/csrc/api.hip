@@ -683,17 +683,29 @@ int mhit_profile_read(mhit_ctx* c, mhit_profile* out) {
     return MHIT_OK;
 }
 
+static void fill_scalars(const mhit_ctx* c, mhit_diag* d) {
+    const DevScalars& S = c->S_host;
+    std::memset(d, 0, sizeof(*d));
+    d->step = S.step;
+    d->alpha = S.alpha; d->beta = S.beta; d->gumax = S.gumax; d->cou = S.cou;
+    d->umean = S.mean[0]; d->vmean = S.mean[1]; d->wmean = S.mean[2];
+}
+
+int mhit_scalars(mhit_ctx* c, mhit_diag* d) {
+    if (!c || !d) { set_error("null argument"); return MHIT_ERR_ARG; }
+    int rc;
+    if ((rc = sync_scalars(c))) return rc;
+    fill_scalars(c, d);
+    return MHIT_OK;
+}
+
 int mhit_diag_compute(mhit_ctx* c, mhit_diag* d) {
     if (!c || !d) { set_error("null argument"); return MHIT_ERR_ARG; }
     int rc;
     if ((rc = sync_scalars(c))) return rc;
     double r[8];
     if ((rc = launch_diag(c, r))) return rc;
-    const DevScalars& S = c->S_host;
-    std::memset(d, 0, sizeof(*d));
-    d->step = S.step;
-    d->alpha = S.alpha; d->beta = S.beta; d->gumax = S.gumax; d->cou = S.cou;
-    d->umean = S.mean[0]; d->vmean = S.mean[1]; d->wmean = S.mean[2];
+    fill_scalars(c, d);
     const double N3 = (double)c->N3;
     d->ke = 0.5 * r[0] / N3;
     d->dissipation = 2.0 * (c->P.mu / c->P.rho) * r[1] / N3;

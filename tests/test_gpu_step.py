@@ -198,3 +198,20 @@ def test_tracer_flag_with_no_tracers(oracle_mod):
     assert max(e.values()) < 1e-12, e
     s.close()
     o.close()
+
+
+def test_scalars_match_diag():
+    """mhit_scalars returns the step's scalars of mhit_diag_compute without
+    the field pass (its diagnostic members stay zero)."""
+    from mhit36_mixedp_amd import Solver
+    print("[step] scalars vs diag, 24^3 multiphase", flush=True)
+    with Solver(nx=24, poisson_precision=1, phiflag=1, partflag=0, dt=5e-4, mu=0.006, sigma=0.5) as s:
+        s.init_random_velocity(5, kmax=4.0, urms=1.0)
+        s.init_drops(2, 0.5, 5)
+        s.step(3)
+        a, d = s.scalars(), s.diag()
+    for k in ("step", "alpha", "beta", "gumax", "cou", "umean", "vmean", "wmean"):
+        assert a[k] == d[k], k
+    assert a["step"] == 3 and a["alpha"] == 1.5 and a["beta"] == 0.5
+    for k in ("ke", "dissipation", "maxdiv", "phi_mass", "nan_count"):
+        assert a[k] == 0, k
