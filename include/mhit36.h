@@ -194,6 +194,8 @@ typedef struct mhit_tuning {
     int32_t ac_mz;        /* z planes per workgroup, fused ACDI pass        */
     int32_t xcd;          /* 1: XCD-aware workgroup order                   */
     int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
+    int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8        */
+    int32_t ac_my;        /* tile rows of the fused ACDI pass: 16 or 8      */
     double ms_per_step;   /* mhit_autotune: measured with these settings    */
 } mhit_tuning;
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);

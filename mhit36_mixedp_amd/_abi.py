@@ -99,6 +99,8 @@ class Tuning(ctypes.Structure):
         ("ac_mz", ctypes.c_int32),
         ("xcd", ctypes.c_int32),
         ("sort_every", ctypes.c_int32),
+        ("ns_my", ctypes.c_int32),
+        ("ac_my", ctypes.c_int32),
         ("ms_per_step", ctypes.c_double),
     ]
 

@@ -71,8 +71,9 @@ static int push_scalars(mhit_ctx* c) {
     return MHIT_OK;
 }
 
-// "key=value,key=value" with keys ns_mz, ac_mz, xcd, sort; unknown keys and
-// non-positive chunk lengths are ignored.
+// "key=value,key=value" with keys ns_mz, ac_mz, ns_my, ac_my, xcd, sort;
+// unknown keys, non-positive chunk lengths and tile rows other than 8 or 16
+// are ignored.
 static void parse_tuning(const char* s, Tuning& t) {
     std::string str(s);
     size_t pos = 0;
@@ -86,11 +87,29 @@ static void parse_tuning(const char* s, Tuning& t) {
             const int val = std::atoi(kv.c_str() + eq + 1);
             if (key == "ns_mz" && val > 0) t.ns_mz = val;
             else if (key == "ac_mz" && val > 0) t.ac_mz = val;
+            else if (key == "ns_my" && (val == 8 || val == 16)) t.ns_my = val;
+            else if (key == "ac_my" && (val == 8 || val == 16)) t.ac_my = val;
             else if (key == "xcd") t.xcd = val != 0;
             else if (key == "sort" && val >= 0) t.sort_every = val;
         }
         pos = end + 1;
     }
+}
+
+// Launch shapes by grid size, then MHIT_TUNE; mhit_autotune refines.
+// z chunks: n/8 within [8, 64] (measured: 64 planes for the 512^3 box, 8
+// for a 64^3 box alone, which needs many workgroups).  Tile rows: 64x8
+// tiles (two workgroups per CU for the ACDI pass) for a small box alone,
+// 64x16 (less halo) for large boxes (measured: ACDI faster with 8 rows up
+// to 128^3, NS up to 64^3) and for boxes batched in an ensemble, where the
+// boxes together fill the GPU and the extra halo of 8-row tiles only costs.
+void default_tuning(mhit_ctx* c, bool batched) {
+    const int n = c->n;
+    c->tune.ns_mz = std::min(64, std::max(8, n / 8));
+    c->tune.ac_mz = std::min(64, std::max(8, n / 8));
+    c->tune.ns_my = (!batched && n <= 64) ? 8 : 16;
+    c->tune.ac_my = (!batched && n <= 128) ? 8 : 16;
+    if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
 }
 
 double* state_field(mhit_ctx* c, int fieldn) {
@@ -331,12 +350,7 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     // MHIT_STENCIL_PATH=cell selects the one-cell-per-thread kernels that
     // follow the reference's loop split (a cross-check of the fused passes)
     if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
-    // default z chunks by grid size, n/8 within [8, 64] (measured: 64 planes
-    // for the 512^3 box; 8 for the 64^3 ensemble boxes, which need many
-    // workgroups per box); mhit_autotune / MHIT_TUNE refine
-    c->tune.ns_mz = std::min(64, std::max(8, n / 8));
-    c->tune.ac_mz = std::min(64, std::max(8, n / 8));
-    if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
+    default_tuning(c, false);
     // MHIT_GRAPH=0 launches the kernels of every step one by one
     if (const char* gr = std::getenv("MHIT_GRAPH")) c->use_graph = std::strcmp(gr, "0") != 0;
     // readinput.f90:39-49

@@ -39,14 +39,17 @@ struct DevScalars {
     int pad_;
 };
 
-// Launch-shape choices that do not change results (z-chunk lengths of the
-// marching passes, XCD-aware tile order).  Defaults are the measured best
-// at 512^3; MHIT_TUNE="ns_mz=64,ac_mz=64,xcd=1,sort=10" overrides them.
+// Launch-shape choices that do not change results (z-chunk lengths and
+// tile rows of the marching passes, XCD-aware tile order).  Defaults are
+// set by grid size at create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,
+// ac_my=8,xcd=1,sort=10" overrides them.
 struct Tuning {
     int ns_mz = 64;       // planes per workgroup, NS pass (mhit_create: n/8 in [8, 64])
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
     int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
+    int ns_my = 16;       // tile rows (16 or 8), NS pass (mhit_create: 8 for n <= 64)
+    int ac_my = 16;       // tile rows (16 or 8), ACDI pass (mhit_create: 8 for n <= 128)
 };
 
 // Per-run constants passed by value to kernels.

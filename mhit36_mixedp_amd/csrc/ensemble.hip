@@ -141,10 +141,11 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
     std::vector<mhit_ctx*> phase;
     for (mhit_ctx* c : boxes)
         if (c->P.phiflag) phase.push_back(c);
-    for (auto& gp : group_by(phase, [](mhit_ctx* c) { return std::make_pair(c->tune.ac_mz, c->tune.xcd); }))
+    auto ac_key = [](mhit_ctx* c) { return std::make_tuple(c->tune.ac_mz, c->tune.ac_my, c->tune.xcd); };
+    for (auto& gp : group_by(phase, ac_key))
         if ((rc = launch_acdi_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
     auto ns_key = [](mhit_ctx* c) {
-        return std::make_tuple(c->P.phiflag != 0, c->fp32, c->tune.ns_mz, c->tune.xcd);
+        return std::make_tuple(c->P.phiflag != 0, c->fp32, c->tune.ns_mz, c->tune.ns_my, c->tune.xcd);
     };
     for (auto& gp : group_by(boxes, ns_key))
         if ((rc = launch_ns_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
@@ -271,6 +272,7 @@ int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble*
             mhit_ensemble_destroy(e);
             return rc;
         }
+        default_tuning(c, true);   // the shapes for batched launches
         e->boxes.push_back(c);
         e->use_graph = e->use_graph && c->use_graph;
     }

@@ -184,6 +184,9 @@ inline void prof_mark(mhit_ctx* c, int s) {
 
 // api.hip: the device array of field id fieldn (1..9 but p), or nullptr
 double* state_field(mhit_ctx* c, int fieldn);
+// api.hip: launch shapes by grid size (for a box alone, or batched in an
+// ensemble), then the MHIT_TUNE overrides
+void default_tuning(mhit_ctx* c, bool batched);
 // dump.hip: wait for a pending dump and free its buffers
 void dump_release(mhit_ctx* c);
 

@@ -47,9 +47,10 @@ __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 // two-point halo on the +x/+y sides for these evaluations, and the fused
 // pass reads its neighbours' previous rhs, so the new AB2 history goes to
 // the other buffer of a ping-pong pair.
-constexpr int NS_MX = 64, NS_MY = 16;  // z chunk: Tuning::ns_mz
-constexpr int NS_W1 = NS_MX + 3, NS_H1 = NS_MY + 3;   // velocity: x, y in [-1, M + 1]
-constexpr int NS_W2 = NS_MX + 4, NS_H2 = NS_MY + 4;   // phi: halo 2
+// Tile: 64 columns in x by TMY rows in y (Tuning::ns_my, 16 or 8), one
+// thread per column; z chunk: Tuning::ns_mz.  The velocity planes carry
+// x, y in [-1, M + 1], phi a two-point halo.
+constexpr int NS_MX = 64;
 
 // Arguments of one box.  A launch covers one box (`one`) or, for an
 // ensemble, a table of boxes of the same size stacked along the grid's z.
@@ -65,8 +66,8 @@ struct NsArgs {
     int zc, xcd;
 };
 
-template <bool PHI, typename TD>
-__global__ __launch_bounds__(NS_MX * NS_MY)
+template <bool PHI, typename TD, int TMY>
+__global__ __launch_bounds__(NS_MX * TMY)
 void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const Blk blk = launch_blk(nbox);
     NsArgs<TD> A = one;
@@ -91,7 +92,7 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const Grid g = A.g;
     const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     const int zc = A.zc, xcd = A.xcd;
-    constexpr int MX = NS_MX, MY = NS_MY, W1 = NS_W1, H1 = NS_H1, W2 = NS_W2, H2 = NS_H2;
+    constexpr int MX = NS_MX, MY = TMY, W1 = MX + 3, H1 = MY + 3, W2 = MX + 4, H2 = MY + 4;
     constexpr int NT = MX * MY;
     constexpr int FXW = MX + 2;                          // fx at x = -1..MX
     __shared__ double su[3][H1 * W1], sv[3][H1 * W1], sw[3][H1 * W1];
@@ -334,9 +335,9 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
 // =====================================================================
 // ACDI
 // =====================================================================
-constexpr int AC_MX = 64, AC_MY = 16;  // z chunk: Tuning::ac_mz
-constexpr int AC_W1 = AC_MX + 2, AC_H1 = AC_MY + 2;   // phi ring: halo 1
-constexpr int AC_W2 = AC_MX + 4, AC_H2 = AC_MY + 4;   // psi ring: halo 2
+// Tile: 64 columns by TMY rows (Tuning::ac_my, 16 or 8); z chunk:
+// Tuning::ac_mz.  phi ring with a one-point halo, psi ring with two.
+constexpr int AC_MX = 64;
 
 struct AcdiArgs {
     double* phi_out;
@@ -347,7 +348,8 @@ struct AcdiArgs {
     int zc, xcd;
 };
 
-__global__ __launch_bounds__(AC_MX * AC_MY)
+template <int TMY>
+__global__ __launch_bounds__(AC_MX * TMY)
 void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
     const Blk blk = launch_blk(nbox);
     AcdiArgs A = one;
@@ -361,7 +363,7 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
     const Grid g = A.g;
     const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     const int zc = A.zc, xcd = A.xcd;
-    constexpr int MX = AC_MX, MY = AC_MY, W1 = AC_W1, H1 = AC_H1, W2 = AC_W2, H2 = AC_H2;
+    constexpr int MX = AC_MX, MY = TMY, W1 = MX + 2, H1 = MY + 2, W2 = MX + 4, H2 = MY + 4;
     constexpr int NT = MX * MY;
     __shared__ double spsi[3][H2 * W2];
     __shared__ double sphi[4][H1 * W1];
@@ -582,8 +584,13 @@ static void ns_swap(mhit_ctx* c) {
 template <bool PH, typename TD>
 static void go_ns(mhit_ctx* c, const NsArgs<TD>& one, const NsArgs<TD>* tab, int nb, hipStream_t st) {
     const int n = c->n;
-    dim3 grid(div_up(n, NS_MX), div_up(n, NS_MY), div_up(n, one.zc) * nb), block(NS_MX, NS_MY, 1);
-    k_ns_march<PH, TD><<<grid, block, 0, st>>>(one, tab, nb);
+    auto go = [&](auto my) {
+        constexpr int MY = decltype(my)::value;
+        dim3 grid(div_up(n, NS_MX), div_up(n, MY), div_up(n, one.zc) * nb), block(NS_MX, MY, 1);
+        k_ns_march<PH, TD, MY><<<grid, block, 0, st>>>(one, tab, nb);
+    };
+    if (c->tune.ns_my == 8) go(std::integral_constant<int, 8>{});
+    else go(std::integral_constant<int, 16>{});
 }
 
 // The divergence goes to the pressure buffer, which the x R2C pass reads
@@ -618,7 +625,8 @@ int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipSt
     mhit_ctx* c = boxes[0];
     for (int b = 1; b < nb; b++)
         if (boxes[b]->n != c->n || boxes[b]->P.phiflag != c->P.phiflag || boxes[b]->fp32 != c->fp32 ||
-            boxes[b]->tune.ns_mz != c->tune.ns_mz || boxes[b]->tune.xcd != c->tune.xcd) {
+            boxes[b]->tune.ns_mz != c->tune.ns_mz || boxes[b]->tune.ns_my != c->tune.ns_my ||
+            boxes[b]->tune.xcd != c->tune.xcd) {
             set_error("batched NS pass: boxes differ in size, phase flag, precision or launch shape");
             return MHIT_ERR_ARG;
         }
@@ -630,8 +638,13 @@ int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipSt
 
 static void go_acdi(mhit_ctx* c, const AcdiArgs& one, const AcdiArgs* tab, int nb, hipStream_t st) {
     const int n = c->n;
-    dim3 grid(div_up(n, AC_MX), div_up(n, AC_MY), div_up(n, one.zc) * nb), block(AC_MX, AC_MY, 1);
-    k_acdi_march<<<grid, block, 0, st>>>(one, tab, nb);
+    auto go = [&](auto my) {
+        constexpr int MY = decltype(my)::value;
+        dim3 grid(div_up(n, AC_MX), div_up(n, MY), div_up(n, one.zc) * nb), block(AC_MX, MY, 1);
+        k_acdi_march<MY><<<grid, block, 0, st>>>(one, tab, nb);
+    };
+    if (c->tune.ac_my == 8) go(std::integral_constant<int, 8>{});
+    else go(std::integral_constant<int, 16>{});
 }
 
 int launch_acdi_march(mhit_ctx* c) {
@@ -647,7 +660,7 @@ int launch_acdi_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hip
     std::vector<AcdiArgs> t(nb);
     for (int b = 0; b < nb; b++) {
         if (boxes[b]->n != c->n || !boxes[b]->P.phiflag || boxes[b]->tune.ac_mz != c->tune.ac_mz ||
-            boxes[b]->tune.xcd != c->tune.xcd) {
+            boxes[b]->tune.ac_my != c->tune.ac_my || boxes[b]->tune.xcd != c->tune.xcd) {
             set_error("batched ACDI pass: boxes differ in size or launch shape, or have no phase field");
             return MHIT_ERR_ARG;
         }

@@ -3,8 +3,9 @@
 // The reference tunes its decomposition at start-up with cuDecomp's
 // autotuner (multi/main.f90:80-136: process grid, transpose and halo
 // backends).  On one GPU the free choices are launch shapes instead: the
-// z planes per workgroup of the two fused marching passes and the
-// XCD-aware tile order.  None of them changes a result bit (every cell is
+// tile rows (64x16 or 64x8) and the z planes per workgroup of the two
+// fused marching passes, and the XCD-aware tile order.  None of them
+// changes a result bit (every cell is
 // computed by the same arithmetic in any shape; the correction's reduction
 // grid is fixed), so the tuner may pick freely.  The tracer re-ordering
 // period (sort_every) is settable but not tuned: its pay-off depends on how
@@ -127,11 +128,15 @@ void to_public(const Tuning& t, mhit_tuning* out, double ms) {
     out->ac_mz = t.ac_mz;
     out->xcd = t.xcd;
     out->sort_every = t.sort_every;
+    out->ns_my = t.ns_my;
+    out->ac_my = t.ac_my;
     out->ms_per_step = ms;
 }
 
 bool valid(const mhit_tuning* t) {
-    return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0;
+    auto rows = [](int r) { return r == 8 || r == 16; };
+    return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
+           rows(t->ns_my) && rows(t->ac_my);
 }
 
 }  // namespace
@@ -150,6 +155,8 @@ int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
     c->tune.ac_mz = t->ac_mz;
     c->tune.xcd = t->xcd;
     c->tune.sort_every = t->sort_every;
+    c->tune.ns_my = t->ns_my;
+    c->tune.ac_my = t->ac_my;
     clear_graphs(c);
     return MHIT_OK;
 }
@@ -173,7 +180,9 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
     };
     struct Param { int* field; std::vector<int> cand; bool active; };
     std::vector<Param> params = {
+        {&c->tune.ns_my, {16, 8}, true},
         {&c->tune.ns_mz, zlist({8, 16, 32, 64, 128}), true},
+        {&c->tune.ac_my, {16, 8}, c->P.phiflag != 0},
         {&c->tune.ac_mz, zlist({16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
     };

@@ -25,6 +25,7 @@ def test_autotune_restores_state_and_results_are_shape_independent():
         print(t, flush=True)
         assert t["ms_per_step"] > 0
         assert t["ns_mz"] in (8, 16, 32) and t["ac_mz"] in (16, 32) and t["xcd"] in (0, 1)
+        assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
         for f in (1, 2, 3, 5, 6, 7, 8, 9):
             assert np.array_equal(a.download_field(f), b.download_field(f)), f
@@ -38,11 +39,17 @@ def test_autotune_restores_state_and_results_are_shape_independent():
         assert a.diag() == b.diag()
 
 
-@pytest.mark.parametrize("shape", [dict(ns_mz=8, ac_mz=16, xcd=1), dict(ns_mz=64, ac_mz=128, xcd=0)])
-def test_set_tuning_is_bitwise_neutral(shape):
+@pytest.mark.parametrize("n,shape", [
+    (64, dict(ns_mz=8, ac_mz=16, xcd=1, ns_my=16, ac_my=16)),
+    (64, dict(ns_mz=64, ac_mz=128, xcd=0, ns_my=8, ac_my=16)),
+    (64, dict(ns_mz=16, ac_mz=8, xcd=0, ns_my=16, ac_my=8)),
+    # 24 rows: the 16-row tiles end in a partial tile (the default is 8)
+    (24, dict(ns_mz=8, ac_mz=8, xcd=0, ns_my=16, ac_my=16)),
+])
+def test_set_tuning_is_bitwise_neutral(n, shape):
     from mhit36_mixedp_amd import Solver
-    print(f"[tuning] {shape}", flush=True)
-    kw = dict(nx=64, phiflag=1, sigma=0.5, dt=5e-4, poisson_precision=1)
+    print(f"[tuning] n={n} {shape}", flush=True)
+    kw = dict(nx=n, phiflag=1, sigma=0.5, dt=5e-4, poisson_precision=1)
     with Solver(**kw) as a, Solver(**kw) as b:
         a.set_tuning(**shape)
         for s in (a, b):
