@@ -280,8 +280,11 @@ int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
  * stopped on cou > 7 (that box stays frozen, the others keep stepping).
  * mhit_ensemble_set_batched: on (the default), boxes of one grid size share
  * every launch of a pass (one launch per pass and size); off, each box
- * runs its own launches on a graph branch of its own.  Results are the
- * same bits either way, and the same as each box stepped alone.           */
+ * runs its own launches on a graph branch of its own.  Either call resets
+ * the boxes' launch shapes to the defaults for that form (long z chunks
+ * when batched, many workgroups per box otherwise; see mhit_tuning).
+ * Results are the same bits either way, and the same as each box stepped
+ * alone.                                                                  */
 typedef struct mhit_ensemble mhit_ensemble;
 int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble** out);
 int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);

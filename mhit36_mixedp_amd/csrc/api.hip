@@ -97,16 +97,20 @@ static void parse_tuning(const char* s, Tuning& t) {
 }
 
 // Launch shapes by grid size, then MHIT_TUNE; mhit_autotune refines.
-// z chunks: n/8 within [8, 64] (measured: 64 planes for the 512^3 box, 8
-// for a 64^3 box alone, which needs many workgroups).  Tile rows: 64x8
-// tiles (two workgroups per CU for the ACDI pass) for a small box alone,
-// 64x16 (less halo) for large boxes (measured: ACDI faster with 8 rows up
-// to 128^3, NS up to 64^3) and for boxes batched in an ensemble, where the
-// boxes together fill the GPU and the extra halo of 8-row tiles only costs.
+// z chunks: for a box alone n/8 within [8, 64] (a small box needs many
+// workgroups) and 128 planes from 512^3 (fewer warm-up planes: 8.17 ->
+// 8.07 ms at 512^3); for boxes batched in an ensemble min(n, 128) (the
+// boxes together fill the GPU: 16.2k -> 16.6k box-steps/s).  Tile rows:
+// 64x8 tiles (two workgroups per CU for the ACDI pass) for a small box
+// alone, 64x16 (less halo) for large boxes (measured: ACDI faster with 8
+// rows up to 128^3, NS up to 64^3) and for boxes batched in an ensemble.
 void default_tuning(mhit_ctx* c, bool batched) {
     const int n = c->n;
-    c->tune.ns_mz = std::min(64, std::max(8, n / 8));
-    c->tune.ac_mz = std::min(64, std::max(8, n / 8));
+    const int zc = batched ? std::min(128, n)
+                 : n >= 512 ? std::min(128, n / 4)
+                            : std::min(64, std::max(8, n / 8));
+    c->tune.ns_mz = zc;
+    c->tune.ac_mz = zc;
     c->tune.ns_my = (!batched && n <= 64) ? 8 : 16;
     c->tune.ac_my = (!batched && n <= 128) ? 8 : 16;
     if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);

@@ -44,7 +44,7 @@ struct DevScalars {
 // set by grid size at create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,
 // ac_my=8,xcd=1,sort=10" overrides them.
 struct Tuning {
-    int ns_mz = 64;       // planes per workgroup, NS pass (mhit_create: n/8 in [8, 64])
+    int ns_mz = 64;       // planes per workgroup, NS pass (set by grid size at create)
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
     int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)

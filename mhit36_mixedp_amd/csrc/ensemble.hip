@@ -26,6 +26,7 @@ int step_async(mhit_ctx* c, int nsteps);     // api.hip: enqueue without waiting
 int finish_steps(mhit_ctx* c);               // api.hip: wait, read the scalars back
 int step_eager_part(mhit_ctx* c);            // api.hip: tracer re-ordering if due
 int step_kernels(mhit_ctx* c);               // api.hip: the kernels of one step
+void clear_graphs(mhit_ctx* c);              // api.hip: drop a box's captured steps
 }
 
 using namespace mhit;
@@ -333,6 +334,12 @@ int mhit_ensemble_set_batched(mhit_ensemble* e, int32_t on) {
     MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
     clear_ensemble_graphs(e);
     e->batched = on != 0;
+    // the launch shapes suited to the mode: a box on its own branch needs
+    // the many-workgroup shapes of a box alone
+    for (mhit_ctx* c : e->boxes) {
+        default_tuning(c, e->batched);
+        clear_graphs(c);
+    }
     return MHIT_OK;
 }
 
