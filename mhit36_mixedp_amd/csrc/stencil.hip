@@ -370,7 +370,10 @@ static CorrArgs<T> corr_args(mhit_ctx* c) {
     a.g = c->g;
     a.S = c->S;
     a.zc = correct_zc(c->n) < c->n ? correct_zc(c->n) : c->n;
-    a.xcd = c->tune.xcd;
+    // XCD-aware tile order always: neighbouring tiles' p halo rows then hit
+    // in the same L2 (1.39 -> 1.36 ms at 512^3; the partials are indexed by
+    // tile, so the order changes no bit)
+    a.xcd = 1;
     return a;
 }
 
@@ -403,8 +406,8 @@ static int correct_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStr
 int launch_correct_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
     mhit_ctx* c = boxes[0];
     for (int b = 1; b < nb; b++)
-        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || boxes[b]->tune.xcd != c->tune.xcd) {
-            set_error("batched correction: boxes differ in size, precision or tile order");
+        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32) {
+            set_error("batched correction: boxes differ in size or precision");
             return MHIT_ERR_ARG;
         }
     return c->fp32 ? correct_batch<float>(boxes, nb, arena, st) : correct_batch<double>(boxes, nb, arena, st);
