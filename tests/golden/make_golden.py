@@ -4,9 +4,10 @@ The reference's own FFT is cuFFT (multi/main.f90:188-205, 853-912), which is
 closed and absent here; its published algorithm is the unnormalised DFT
 X[k] = sum_j x[j] exp(-2 pi i jk/N) (forward) and exp(+2 pi i jk/N)
 (inverse).  numpy.fft (pocketfft) computes the same DFT, so it writes the
-vectors.  The validation curves val/cudec_128_ab.txt and
-val/single_256_ab.txt (TGV dissipation vs t at Re 1600, 128^3 and 256^3,
-AB2) are copied from the reference as data (tests/test_gpu_validation.py).
+vectors.  The validation curves val/cudec_128_ab.txt, val/single_128_ab.txt
+and val/single_256_ab.txt (TGV dissipation vs t at Re 1600, 128^3 and
+256^3, AB2) are copied from the reference as data
+(tests/test_gpu_validation.py).
 
 Run:  python tests/golden/make_golden.py   (writes fft_golden.npz)
 """

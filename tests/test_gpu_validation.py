@@ -1,7 +1,8 @@
 """End to end against the reference's own validation data: the decaying
 Taylor-Green vortex at Re = 1600 (nu = 6.25e-4, dt = 2e-4, AB2, no forcing),
-whose dissipation curve the reference ships as val/cudec_128_ab.txt (128^3,
-copied into tests/golden as data).  The reference's dissipation formula is
+whose dissipation curves the reference ships as val/cudec_128_ab.txt
+(128^3, FP64) and val/single_128_ab.txt (FP32 Poisson), copied into
+tests/golden as data.  The reference's dissipation formula is
 not given with the curve; the solver's is 2 nu <S_ij S_ij> with central
 differences (post/compute_diss).  Measured agreement at 128^3: within 4.3%
 up to the dissipation peak at t ~ 8.5 (0.3% at t = 0, peak value within
@@ -26,14 +27,17 @@ def _curve(name):
     return np.array(rows)
 
 
-def test_tgv_dissipation_curve_128():
+@pytest.mark.parametrize("prec,curve", [(0, "val_cudec_128_ab.txt"), (1, "val_single_128_ab.txt")])
+def test_tgv_dissipation_curve_128(prec, curve):
+    """FP64 Poisson against the reference's FP64 curve, FP32 Poisson
+    against its single-precision curve (val/single_128_ab.txt)."""
     from mhit36_mixedp_amd import Solver
-    print("[validation] TGV Re 1600, 128^3, 50000 steps", flush=True)
-    ref = _curve("val_cudec_128_ab.txt")
+    print(f"[validation] TGV Re 1600, 128^3, 50000 steps, poisson_precision={prec}", flush=True)
+    ref = _curve(curve)
     dt = 2e-4
     per = int(round(0.2 / dt))
     eps = []
-    with Solver(nx=128, poisson_precision=0, phiflag=0, partflag=0, dt=dt, mu=6.25e-4, rho=1.0,
+    with Solver(nx=128, poisson_precision=prec, phiflag=0, partflag=0, dt=dt, mu=6.25e-4, rho=1.0,
                 f1=0.0, f2=0.0, f3=0.0) as s:
         s.init_taylor_green()
         eps.append(s.diag()["dissipation"])
