@@ -170,7 +170,7 @@ double stage_bytes(const mhit_ctx* c, int s) {
 // is not part of it)
 static int enqueue_step_kernels(mhit_ctx* c) {
     int rc;
-    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c))) return rc;   // 469-510
+    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c, c->stream))) return rc; // 469-510
     if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
     if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
     if ((rc = poisson_project(c))) return rc;                                // 833-923

@@ -132,13 +132,7 @@ static std::vector<std::vector<mhit_ctx*>> group_by(const std::vector<mhit_ctx*>
 static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar, hipStream_t st) {
     int rc;
     for (mhit_ctx* c : boxes)
-        if (c->P.partflag && c->np > 0) {
-            hipStream_t own = c->stream;
-            c->stream = st;
-            rc = launch_tracers(c);
-            c->stream = own;
-            if (rc) return rc;
-        }
+        if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c, st))) return rc;
     std::vector<mhit_ctx*> phase;
     for (mhit_ctx* c : boxes)
         if (c->P.phiflag) phase.push_back(c);

@@ -227,7 +227,7 @@ int launch_acdi_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hip
 int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st);
 
 // tracers.hip
-int launch_tracers(mhit_ctx* c);
+int launch_tracers(mhit_ctx* c, hipStream_t st);
 int sort_tracers(mhit_ctx* c);          // stable cell-order (Morton) permutation
 void free_sort_buffers(mhit_ctx* c);
 

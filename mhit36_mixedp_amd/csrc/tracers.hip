@@ -179,14 +179,14 @@ int sort_tracers(mhit_ctx* c) {
     return MHIT_OK;
 }
 
-int launch_tracers(mhit_ctx* c) {
+int launch_tracers(mhit_ctx* c, hipStream_t st) {
     if (c->np <= 0) return MHIT_OK;
     const int blocks = div_up(c->np, 256);
     if (c->P.interp_mode == MHIT_INTERP_TRILINEAR)
-        k_tracers<MHIT_INTERP_TRILINEAR><<<blocks, 256, 0, c->stream>>>(
+        k_tracers<MHIT_INTERP_TRILINEAR><<<blocks, 256, 0, st>>>(
             c->u, c->v, c->w, c->px, c->py, c->pz, c->pu1, c->pv1, c->pw1, c->np, c->g, c->S);
     else
-        k_tracers<MHIT_INTERP_REFERENCE><<<blocks, 256, 0, c->stream>>>(
+        k_tracers<MHIT_INTERP_REFERENCE><<<blocks, 256, 0, st>>>(
             c->u, c->v, c->w, c->px, c->py, c->pz, c->pu1, c->pv1, c->pw1, c->np, c->g, c->S);
     prof_mark(c, ST_TRACERS);
     MHIT_HIP_TRY(hipGetLastError());
