@@ -81,6 +81,24 @@ def measured_traffic(stage: str, n: int):
     return None if k is None else float(k["read_bytes"] + k["write_bytes"])
 
 
+def host_cpu():
+    """The host processor model and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "nproc": usable, "cpu_count": os.cpu_count()}
+
+
 def cpu_baseline(args, fields, tracers):
     """Time the CPU oracle on one step of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -327,7 +345,8 @@ def main():
         cpu = {"value": round(cpu_ms, 3), "unit": "ms/timestep", "cores": threads, "kind": "port",
                "sample": f"{args.cpu_steps} timestep(s) of the same {args.n}^3 multiphase workload "
                          f"({args.ndrops} drops, {args.ntracers} tracers, FP32 Poisson) on the CPU "
-                         "FP64 restatement (oracle/), OpenMP"}
+                         "FP64 restatement (oracle/), OpenMP",
+               "host": host_cpu()}
 
     small = None
     if not args.no_cpu_baseline:
