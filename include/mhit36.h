@@ -281,8 +281,10 @@ int mhit_fft_c2r(mhit_ctx* ctx, const double* in, double* out);
  * mhit_ensemble_set_batched: on (the default), boxes of one grid size share
  * every launch of a pass (one launch per pass and size); off, each box
  * runs its own launches on a graph branch of its own.  Either call resets
- * the boxes' launch shapes to the defaults for that form (long z chunks
- * when batched, many workgroups per box otherwise; see mhit_tuning).
+ * the boxes' launch shapes to the defaults for that form (batched: z
+ * chunks of up to 128 planes, shortened until the batched launches hold
+ * about four workgroups per CU; otherwise many workgroups per box; see
+ * mhit_tuning).
  * Results are the same bits either way, and the same as each box stepped
  * alone.                                                                  */
 typedef struct mhit_ensemble mhit_ensemble;

@@ -99,12 +99,12 @@ static void parse_tuning(const char* s, Tuning& t) {
 // Launch shapes by grid size, then MHIT_TUNE; mhit_autotune refines.
 // z chunks: for a box alone n/8 within [8, 64] (a small box needs many
 // workgroups) and 128 planes from 512^3 (fewer warm-up planes: 8.17 ->
-// 8.07 ms at 512^3); for boxes batched in an ensemble min(n, 128) (the
-// boxes together fill the GPU: 16.2k -> 16.6k box-steps/s).  Tile rows:
+// 8.07 ms at 512^3); for boxes batched in an ensemble min(n, 128), which
+// the ensemble then shortens until its launches fill the GPU.  Tile rows:
 // 64x8 tiles (two workgroups per CU for the ACDI pass) for a small box
 // alone, 64x16 (less halo) for large boxes (measured: ACDI faster with 8
 // rows up to 128^3, NS up to 64^3) and for boxes batched in an ensemble.
-void default_tuning(mhit_ctx* c, bool batched) {
+void default_tuning(mhit_ctx* c, bool batched, bool env) {
     const int n = c->n;
     const int zc = batched ? std::min(128, n)
                  : n >= 512 ? std::min(128, n / 4)
@@ -113,6 +113,10 @@ void default_tuning(mhit_ctx* c, bool batched) {
     c->tune.ac_mz = zc;
     c->tune.ns_my = (!batched && n <= 64) ? 8 : 16;
     c->tune.ac_my = (!batched && n <= 128) ? 8 : 16;
+    if (env) tuning_env(c);
+}
+
+void tuning_env(mhit_ctx* c) {
     if (const char* t = std::getenv("MHIT_TUNE")) parse_tuning(t, c->tune);
 }
 

@@ -186,7 +186,8 @@ inline void prof_mark(mhit_ctx* c, int s) {
 double* state_field(mhit_ctx* c, int fieldn);
 // api.hip: launch shapes by grid size (for a box alone, or batched in an
 // ensemble), then the MHIT_TUNE overrides
-void default_tuning(mhit_ctx* c, bool batched);
+void default_tuning(mhit_ctx* c, bool batched, bool env = true);
+void tuning_env(mhit_ctx* c);   // the MHIT_TUNE overrides alone
 // dump.hip: wait for a pending dump and free its buffers
 void dump_release(mhit_ctx* c);
 

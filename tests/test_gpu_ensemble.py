@@ -92,3 +92,46 @@ def test_ensemble_cfl_stop_isolated(batched):
             ens.step(2)
         assert ens.boxes[1].diag()["step"] == 1
         assert ens.boxes[0].diag()["step"] == 5
+
+
+def _ns_workgroups(boxes):
+    total = 0
+    for b in boxes:
+        n, t = b.n, b.tuning()
+        total += -(-n // 64) * -(-n // t["ns_my"]) * -(-n // t["ns_mz"])
+    return total
+
+
+def test_batched_shapes_fill_the_gpu():
+    """Batched boxes get z chunks halved from min(n, 128) (never below 8
+    planes) until the NS launches of one step hold about four workgroups
+    per CU (on a 256-CU MI355X: 32 planes for 16 boxes of 128^3, 16 for 8
+    boxes); unbatched boxes get the shapes of a box alone.  Checked without
+    the CU count: half the boxes get half the chunk and the same number of
+    workgroups."""
+    from mhit36_mixedp_amd import Ensemble, Solver
+    print("[ensemble] batched launch shapes, 16 and 8 boxes of 128^3", flush=True)
+    chosen = {}
+    for nbox in (16, 8):
+        params = [_params(dict(nx=128, phiflag=i % 2, partflag=0, sigma=0.5)) for i in range(nbox)]
+        with Ensemble(params) as ens:
+            t = [b.tuning() for b in ens.boxes]
+            assert len({(x["ns_mz"], x["ac_mz"], x["ns_my"]) for x in t}) == 1
+            zc = t[0]["ns_mz"]
+            assert zc == t[0]["ac_mz"] and t[0]["ns_my"] == 16 and 8 <= zc <= 128 and 128 % zc == 0
+            chosen[nbox] = (zc, _ns_workgroups(ens.boxes))
+            if nbox == 8:
+                with Solver(_params(dict(nx=128, phiflag=0, partflag=0))) as solo:
+                    alone = solo.tuning()
+                ens.set_batched(False)
+                for b in ens.boxes:
+                    x = b.tuning()
+                    assert (x["ns_mz"], x["ns_my"], x["ac_my"]) == (alone["ns_mz"], alone["ns_my"], alone["ac_my"])
+                ens.set_batched(True)
+                assert all(b.tuning()["ns_mz"] == zc for b in ens.boxes)
+    (z16, w16), (z8, w8) = chosen[16], chosen[8]
+    print(chosen, flush=True)
+    if z16 < 128 and z8 > 8:
+        # halving stops at the first workgroup count that reaches the
+        # target: the same count for 16 boxes and for 8
+        assert z8 * 2 == z16 and w8 == w16
