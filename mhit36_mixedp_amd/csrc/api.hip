@@ -97,22 +97,32 @@ static void parse_tuning(const char* s, Tuning& t) {
 }
 
 // Launch shapes by grid size, then MHIT_TUNE; mhit_autotune refines.
-// z chunks: for a box alone n/8 within [8, 64] (a small box needs many
-// workgroups) and 128 planes from 512^3 (fewer warm-up planes: 8.17 ->
-// 8.07 ms at 512^3); for boxes batched in an ensemble min(n, 128), which
-// the ensemble then shortens until its launches fill the GPU.  Tile rows:
-// 64x8 tiles (two workgroups per CU for the ACDI pass) for a small box
-// alone, 64x16 (less halo) for large boxes (measured: ACDI faster with 8
-// rows up to 128^3, NS up to 64^3) and for boxes batched in an ensemble.
+// A box alone: 64x8 tiles up to 64^3, 64x16 above (less halo); z chunks
+// of min(128, n/4) planes, halved (never below 4) until the NS pass has a
+// workgroup per CU.  Measured, ms per step, FP32 Poisson, single phase /
+// multiphase: 64^3 0.075 / 0.113 at 8 planes -> 0.066 / 0.091 at 4;
+// 128^3 0.137 / 0.206 at 16 planes (64x8 ACDI tiles) -> 0.124 / 0.175 at
+// 8; 256^3 0.700 / 1.021 at 32 -> 0.674 / 0.983 at 64; 512^3 keeps 128
+// (fewer warm-up planes than 64: 8.17 -> 8.07 ms).  Boxes batched in an
+// ensemble: 64x16 tiles and min(n, 128) planes, which the ensemble then
+// shortens until its launches fill the GPU.
 void default_tuning(mhit_ctx* c, bool batched, bool env) {
     const int n = c->n;
-    const int zc = batched ? std::min(128, n)
-                 : n >= 512 ? std::min(128, n / 4)
-                            : std::min(64, std::max(8, n / 8));
+    const int rows = (!batched && n <= 64) ? 8 : 16;
+    int zc = std::min(128, n);
+    if (!batched) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+            cus <= 0)
+            cus = 256;
+        zc = std::min(128, std::max(4, n / 4));
+        const long long tiles = (long long)div_up(n, 64) * div_up(n, rows);
+        while (tiles * div_up(n, zc) < cus && zc / 2 >= 4) zc /= 2;
+    }
     c->tune.ns_mz = zc;
     c->tune.ac_mz = zc;
-    c->tune.ns_my = (!batched && n <= 64) ? 8 : 16;
-    c->tune.ac_my = (!batched && n <= 128) ? 8 : 16;
+    c->tune.ns_my = rows;
+    c->tune.ac_my = rows;
     if (env) tuning_env(c);
 }
 

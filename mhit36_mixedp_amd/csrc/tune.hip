@@ -181,9 +181,9 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
     struct Param { int* field; std::vector<int> cand; bool active; };
     std::vector<Param> params = {
         {&c->tune.ns_my, {16, 8}, true},
-        {&c->tune.ns_mz, zlist({8, 16, 32, 64, 128}), true},
+        {&c->tune.ns_mz, zlist({4, 8, 16, 32, 64, 128}), true},
         {&c->tune.ac_my, {16, 8}, c->P.phiflag != 0},
-        {&c->tune.ac_mz, zlist({16, 32, 64, 128}), c->P.phiflag != 0},
+        {&c->tune.ac_mz, zlist({4, 8, 16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
     };
     double best_ms = 0.0;

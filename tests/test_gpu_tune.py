@@ -24,7 +24,7 @@ def test_autotune_restores_state_and_results_are_shape_independent():
         t = a.autotune(steps=2)
         print(t, flush=True)
         assert t["ms_per_step"] > 0
-        assert t["ns_mz"] in (8, 16, 32) and t["ac_mz"] in (16, 32) and t["xcd"] in (0, 1)
+        assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
         assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
         for f in (1, 2, 3, 5, 6, 7, 8, 9):
@@ -43,6 +43,9 @@ def test_autotune_restores_state_and_results_are_shape_independent():
     (64, dict(ns_mz=8, ac_mz=16, xcd=1, ns_my=16, ac_my=16)),
     (64, dict(ns_mz=64, ac_mz=128, xcd=0, ns_my=8, ac_my=16)),
     (64, dict(ns_mz=16, ac_mz=8, xcd=0, ns_my=16, ac_my=8)),
+    # 4-plane chunks: warm-up planes longer than the chunk
+    (64, dict(ns_mz=4, ac_mz=4, xcd=0, ns_my=8, ac_my=8)),
+    (48, dict(ns_mz=4, ac_mz=4, xcd=1, ns_my=16, ac_my=16)),
     # 24 rows: the 16-row tiles end in a partial tile (the default is 8)
     (24, dict(ns_mz=8, ac_mz=8, xcd=0, ns_my=16, ac_my=16)),
 ])
