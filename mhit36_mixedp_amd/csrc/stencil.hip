@@ -237,17 +237,24 @@ int launch_ns(mhit_ctx* c) {
 // ----------------------------------------------------------- correction
 // C, main.f90:942-962: u = u* - dt/rho (p - p(-1)) dxi with the pressure
 // promoted to FP64, plus per-block sums and maxima of the corrected u,v,w.
-// A workgroup owns a 64 x 8 tile of columns and marches a chunk of 32 z
-// planes: p(z-1) of its column stays in a register, the plane's p with a
-// one-point halo at -x and -y is staged in LDS, and the next plane is
-// fetched into registers during the update.  The number of partials is
+// A workgroup owns a 64 x 8 tile of columns and marches a chunk of z
+// planes (correct_zc): p(z-1) of its column stays in a register, the
+// plane's p with a one-point halo at -x and -y is staged in LDS, and the
+// next plane is fetched into registers during the update.  The number of partials is
 // fixed by the grid, so the reduction order is deterministic.
 constexpr int kCorrZ = 16;                    // z split of the diagnostics pass
 constexpr int CO_MX = 64, CO_MY = 8;
-// z planes per workgroup: a function of n only (the number of partials, and
-// so the reduction order, is fixed per grid size); small boxes get short
-// chunks so that a box still spreads over many workgroups
-static int correct_zc(int n) { return n >= 256 ? 32 : (n >= 128 ? 16 : 8); }
+// z planes per workgroup: the longest of 32, 16, ..., 2 that gives a box
+// at least 256 workgroups (512^3 and 256^3: 32, 128^3: 16, 96^3: 8, 64^3
+// and below: 2; at 64^3 8-plane chunks left 192 CUs idle, 14.8 us).  A
+// function of n only, never of the device: the number of partials, and so
+// the reduction order, is fixed per grid size.
+static int correct_zc(int n) {
+    const long long tiles = (long long)div_up(n, CO_MX) * div_up(n, CO_MY);
+    int zc = 32;
+    while (zc > 2 && tiles * div_up(n, zc) < 256) zc /= 2;
+    return zc;
+}
 
 static dim3 correct_grid(int n) {
     const int zc = correct_zc(n) < n ? correct_zc(n) : n;
