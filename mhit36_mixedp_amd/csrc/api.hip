@@ -368,7 +368,6 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
     // MHIT_STENCIL_PATH=cell selects the one-cell-per-thread kernels that
     // follow the reference's loop split (a cross-check of the fused passes)
     if (const char* sp = std::getenv("MHIT_STENCIL_PATH")) c->march = std::strcmp(sp, "cell") != 0;
-    default_tuning(c, false);
     // MHIT_GRAPH=0 launches the kernels of every step one by one
     if (const char* gr = std::getenv("MHIT_GRAPH")) c->use_graph = std::strcmp(gr, "0") != 0;
     // readinput.f90:39-49
@@ -391,6 +390,7 @@ int mhit_create(const mhit_params* P, mhit_ctx** out) {
 
     auto fail = [&](int code) { mhit_destroy(c); return code; };
     if (hipGetDevice(&c->device) != hipSuccess) { set_error("hipGetDevice failed"); return fail(MHIT_ERR_HIP); }
+    default_tuning(c, false);   // launch shapes for this device's CU count
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         return fail(MHIT_ERR_HIP);
