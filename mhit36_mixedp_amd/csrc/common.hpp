@@ -49,7 +49,7 @@ struct Tuning {
     int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
     int ns_my = 16;       // tile rows (16 or 8), NS pass (mhit_create: 8 for n <= 64)
-    int ac_my = 16;       // tile rows (16 or 8), ACDI pass (mhit_create: 8 for n <= 128)
+    int ac_my = 16;       // tile rows (16 or 8), ACDI pass (mhit_create: 8 for n <= 64)
 };
 
 // Per-run constants passed by value to kernels.
