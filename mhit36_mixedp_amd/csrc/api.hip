@@ -184,12 +184,27 @@ double stage_bytes(const mhit_ctx* c, int s) {
 // is not part of it)
 static int enqueue_step_kernels(mhit_ctx* c) {
     int rc;
-    if (c->P.partflag && c->np > 0 && (rc = launch_tracers(c, c->stream))) return rc; // 469-510
-    if (c->P.phiflag && (rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
-    if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;      // 658-796
-    if ((rc = poisson_project(c))) return rc;                                // 833-923
-    if ((rc = launch_correct(c))) return rc;                                 // 942-962
-    if ((rc = launch_finalize(c))) return rc;                                // 964-1007
+    if (c->P.partflag && c->np > 0) {
+        Range r("Particles");
+        if ((rc = launch_tracers(c, c->stream))) return rc;                  // 469-510
+    }
+    if (c->P.phiflag) {
+        Range r("Phase-field");
+        if ((rc = c->march ? launch_acdi_march(c) : launch_acdi(c))) return rc; // 521-637
+    }
+    {
+        Range r("Projection");
+        if ((rc = c->march ? launch_ns_march(c) : launch_ns(c))) return rc;  // 658-796
+    }
+    {
+        Range r("Poisson");
+        if ((rc = poisson_project(c))) return rc;                            // 833-923
+    }
+    {
+        Range r("Correction");
+        if ((rc = launch_correct(c))) return rc;                             // 942-962
+        if ((rc = launch_finalize(c))) return rc;                            // 964-1007
+    }
     return MHIT_OK;
 }
 
@@ -265,6 +280,7 @@ int step_kernels(mhit_ctx* c) { return enqueue_step_kernels(c); }
 // one timestep, main.f90:456-1007
 static int enqueue_step(mhit_ctx* c) {
     int rc;
+    Range r("Iteration");
     if (c->prof) {
         c->prof_n = 0;
         prof_mark(c, -1);

@@ -351,8 +351,10 @@ int mhit_ensemble_step(mhit_ensemble* e, int nsteps) {
             MHIT_HIP_TRY(hipEventRecord(e->join[b], e->boxes[b]->stream));
             MHIT_HIP_TRY(hipStreamWaitEvent(e->stream, e->join[b], 0));
         }
-        for (int s = 0; s < nsteps; s++)
+        for (int s = 0; s < nsteps; s++) {
+            Range r("Iteration");
             if ((rc = ensemble_graph_step(e))) return rc;
+        }
         MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
     } else {
         // one stream per box, interleaved step by step so that no stream

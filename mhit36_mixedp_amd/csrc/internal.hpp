@@ -4,6 +4,8 @@
 
 #include "common.hpp"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include <cstring>
 #include <utility>
 #include <vector>
@@ -173,6 +175,20 @@ inline hipError_t fill_on_stream(mhit_ctx* c, void* dst, int value, size_t bytes
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     return e;
 }
+
+// roctx ranges around the host-side enqueue of a timestep and of its
+// stages, named as the reference's NVTX ranges (commented out there:
+// "Iteration" main.f90:460, "Phase-field" 517, "Projection" 655, "Poisson"
+// 831, "Correction" 940; "Particles" has no counterpart).  Under rocprofv3
+// --marker-trace they group the launches; with no tool attached a push/pop
+// pair costs tens of nanoseconds.  A captured step replays inside its
+// "Iteration" range only (its stages were enqueued once, at capture).
+struct Range {
+    explicit Range(const char* name) { roctxRangePushA(name); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range&) = delete;
+    Range& operator=(const Range&) = delete;
+};
 
 // Record the end of stage s on the context stream (profiling only).
 inline void prof_mark(mhit_ctx* c, int s) {

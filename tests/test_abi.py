@@ -39,6 +39,20 @@ def test_every_declared_symbol_is_exported(lib):
     assert sorted(n for n, _, _ in _abi.EXPORTS) == names
 
 
+def test_step_stages_carry_roctx_ranges(lib):
+    """The timestep and its stages are wrapped in roctx ranges (the
+    reference's NVTX ranges, main.f90:460-1040), so the library imports
+    the roctx push/pop pair and names the reference's range labels."""
+    path = os.path.join(ROOT, "mhit36_mixedp_amd", "libmhit36.so")
+    und = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True,
+                         check=True).stdout
+    assert "roctxRangePushA" in und and "roctxRangePop" in und
+    with open(path, "rb") as f:
+        blob = f.read()
+    for label in (b"Iteration", b"Phase-field", b"Projection", b"Poisson", b"Correction"):
+        assert label + b"\0" in blob, label
+
+
 def test_struct_layout_matches_c():
     from mhit36_mixedp_amd import _abi
     src = r"""
