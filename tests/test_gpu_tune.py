@@ -61,3 +61,22 @@ def test_set_tuning_is_bitwise_neutral(n, shape):
             s.step(4)
         for f in (1, 2, 3, 4, 5):
             assert np.array_equal(a.download_field(f), b.download_field(f)), f
+
+
+def test_mhit_tune_overrides_and_ignores_bad_values(monkeypatch):
+    """MHIT_TUNE is read at context creation: valid keys override the
+    size-based shapes, unknown keys, non-positive chunks and tile rows
+    other than 8 or 16 are ignored."""
+    from mhit36_mixedp_amd import Solver
+    print("[tuning] MHIT_TUNE parsing", flush=True)
+    with Solver(nx=32, phiflag=1, sigma=0.5) as s:
+        base = s.tuning()
+    monkeypatch.setenv("MHIT_TUNE", "ns_mz=16,ac_mz=8,ns_my=16,ac_my=16,xcd=1,sort=3")
+    with Solver(nx=32, phiflag=1, sigma=0.5) as s:
+        t = s.tuning()
+    assert (t["ns_mz"], t["ac_mz"], t["ns_my"], t["ac_my"], t["xcd"], t["sort_every"]) == (16, 8, 16, 16, 1, 3)
+    monkeypatch.setenv("MHIT_TUNE", "ns_mz=0,ac_mz=-4,ns_my=12,ac_my=4,bogus=7,sort=-1")
+    with Solver(nx=32, phiflag=1, sigma=0.5) as s:
+        t = s.tuning()
+    for k in ("ns_mz", "ac_mz", "ns_my", "ac_my", "xcd", "sort_every"):
+        assert t[k] == base[k], (k, t[k], base[k])
