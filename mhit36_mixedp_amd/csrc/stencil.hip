@@ -245,11 +245,14 @@ int launch_ns(mhit_ctx* c) {
 constexpr int kCorrZ = 16;                    // z split of the diagnostics pass
 constexpr int CO_MX = 64, CO_MY = 8;
 // z planes per workgroup: the longest of 32, 16, ..., 2 that gives a box
-// at least 256 workgroups (512^3 and 256^3: 32, 128^3: 16, 96^3: 8, 64^3
-// and below: 2; at 64^3 8-plane chunks left 192 CUs idle, 14.8 us).  A
+// at least 256 workgroups (256^3: 32, 128^3: 16, 96^3: 8, 64^3 and below:
+// 2; at 64^3 8-plane chunks left 192 CUs idle, 14.8 us), and 16 from
+// 512^3 (1.36 -> 1.31 ms; 8 planes gain no more once the larger finalize
+// is counted; at 256^3 16 and 8 planes measured the same as 32).  A
 // function of n only, never of the device: the number of partials, and so
 // the reduction order, is fixed per grid size.
 static int correct_zc(int n) {
+    if (n >= 512) return 16;
     const long long tiles = (long long)div_up(n, CO_MX) * div_up(n, CO_MY);
     int zc = 32;
     while (zc > 2 && tiles * div_up(n, zc) < 256) zc /= 2;
