@@ -171,6 +171,29 @@ def small_box_comparison(args):
 ENSEMBLE_SIZES = (64, 96, 128)
 
 
+def one_size_rates(args):
+    """Box-steps/s of 16 boxes of one size (every other box multiphase),
+    batched, for each size of the ensemble workload: the case where one
+    size class alone has to fill the GPU."""
+    from mhit36_mixedp_amd import Ensemble, default_params
+    out = {}
+    for n in ENSEMBLE_SIZES:
+        params = [default_params(nx=n, phiflag=i % 2, partflag=0, poisson_precision=1, dt=5e-4,
+                                 mu=0.004 + 0.0005 * (i % 9), rho=1.0, sigma=2 * 0.4, epsr=1.0)
+                  for i in range(16)]
+        with Ensemble(params) as ens:
+            for i, b in enumerate(ens.boxes):
+                b.init_random_velocity(args.seed + i, kmax=4.0, urms=1.0)
+                if i % 2:
+                    b.init_drops(1 + i % 6, 0.4, args.seed + i)
+            ens.step(args.warmup)
+            t0 = time.perf_counter()
+            ens.step(args.ensemble_steps)
+            dt = (time.perf_counter() - t0) / args.ensemble_steps
+        out[f"16x{n}^3"] = round(len(params) / dt, 1)
+    return out
+
+
 def ensemble_leg(args, batched_only=False):
     """Ensemble workload: 16 boxes each of 64^3, 96^3 and 128^3 (51,904,512
     cells), every other box multiphase, each with its own seed, viscosity
@@ -229,6 +252,7 @@ def ensemble_leg(args, batched_only=False):
         "hbm_GBps": round(nbytes / together / 1e9, 1),
         "branch_per_box_box_steps_per_s": round(nbox / branches, 1),
         "one_at_a_time_box_steps_per_s": round(nbox / alone, 1),
+        "one_size_box_steps_per_s": one_size_rates(args),
     }
 
 
