@@ -102,7 +102,6 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     __shared__ double sus[MY * (MX + 1)];               // u* at x = 0..MX
     __shared__ double svs[(MY + 1) * MX];               // v* at y = 0..MY
 
-    if (S->halted) return;
     const int n = g.n;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * MX + tx;
     int tbx, tby, tbz;
@@ -112,6 +111,19 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
+    if (S->halted) {
+        // a stopped run keeps its state: the new AB2 history becomes a copy
+        // of the old one, so the host-side swap of the two buffers leaves
+        // the history of the last step taken in place
+        if (live)
+            for (int k = z0; k < z1; k++) {
+                const size_t q = idx3(i, j, k, n);
+                ru_n[q] = ru_o[q];
+                rv_n[q] = rv_o[q];
+                rw_n[q] = rw_o[q];
+            }
+        return;
+    }
     // global indices of the cell, or of its periodic image past the domain edge
     const int ic = wrapn(i, n), jc = wrapn(j, n);
     const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];

@@ -33,13 +33,18 @@ using namespace mhit;
 namespace {
 
 // Device copy of everything a timestep reads, by role (the ping-pong and
-// re-ordering swap buffers, so restore goes to the current owner of a role).
+// re-ordering swap buffers, so restore goes to the current owner of a role),
+// and of the pressure, which a download or dump returns between steps.
 struct Snapshot {
     std::vector<double*> copies;
     std::vector<size_t> counts;
+    void* pres = nullptr;   // the last projection's pressure, in its own precision
+    size_t pres_bytes = 0;
     DevScalars S{};
     long long steps_since_sort = 0;
 };
+
+size_t pres_bytes(const mhit_ctx* c) { return c->N3 * (c->fp32 ? sizeof(float) : sizeof(double)); }
 
 std::vector<double**> roles(mhit_ctx* c) {
     std::vector<double**> r = {&c->u, &c->v, &c->w, &c->rhsu_o, &c->rhsv_o, &c->rhsw_o};
@@ -55,6 +60,8 @@ size_t role_count(mhit_ctx* c, size_t k) { return k < (c->P.phiflag ? 8u : 6u) ?
 void release(Snapshot& s) {
     for (double* p : s.copies) (void)hipFree(p);
     s.copies.clear();
+    if (s.pres) (void)hipFree(s.pres);
+    s.pres = nullptr;
 }
 
 int take(mhit_ctx* c, Snapshot& s) {
@@ -79,6 +86,18 @@ int take(mhit_ctx* c, Snapshot& s) {
             return MHIT_ERR_HIP;
         }
     }
+    s.pres_bytes = pres_bytes(c);
+    if (hipMalloc(&s.pres, s.pres_bytes) != hipSuccess) {
+        s.pres = nullptr;
+        release(s);
+        set_error("autotune: no device memory for the state snapshot");
+        return MHIT_ERR_NOMEM;
+    }
+    if (hipMemcpyAsync(s.pres, c->pres, s.pres_bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        release(s);
+        set_error("autotune: state snapshot copy failed");
+        return MHIT_ERR_HIP;
+    }
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         release(s);
         set_error("autotune: state snapshot copy failed");
@@ -92,6 +111,7 @@ int restore(mhit_ctx* c, const Snapshot& s) {
     for (size_t k = 0; k < r.size(); k++)
         MHIT_HIP_TRY(hipMemcpyAsync(*r[k], s.copies[k], s.counts[k] * sizeof(double),
                                     hipMemcpyDeviceToDevice, c->stream));
+    MHIT_HIP_TRY(hipMemcpyAsync(c->pres, s.pres, s.pres_bytes, hipMemcpyDeviceToDevice, c->stream));
     c->S_host = s.S;
     c->steps_since_sort = s.steps_since_sort;
     int rc;

@@ -1,0 +1,167 @@
+"""The five workloads the solver is specified on, run on the GPU with the
+bars they are specified with.
+
+C1  64^3 single-phase NS in FP64: seeded random-phase k^-2 field over
+    |k| <= 8, ABC forcing A = B = C = 1 at k0 = 2, nu = 0.006, dt = 5e-4, AB2,
+    100 steps.  KE, dissipation and max |div u| of every step, and the
+    fields after the last, against the CPU oracle (FP64 restatement of
+    multi/main.f90:456-1007) within 1e-10 relative.
+C2  256^3 single phase, nu = 0.002, dt = 2e-4, 1000 steps: the FP32-Poisson
+    form against the FP64-Poisson form (the FP64 GPU oracle) from the same
+    seeded field.  KE within 1e-6 relative; with the modified wavenumber
+    k'^2 max |div u| <= 1e-5 (with the reference's continuous k^2 the
+    projection is not discretely exact and the divergence is that of the
+    reference algorithm in either precision).
+C3  256^3 + 64 drops of R = 0.2 (eps = dx, We = 1) + 1,048,576 tracers,
+    1000 steps: KE, phi-mass drift, tracer positions (matched by tag)
+    against the FP64 form; max |div u| <= 1e-5 with k'^2.
+C4  512^3 + 512 drops of R = 0.1 + 4,194,304 tracers, dt = 1e-4, 1000 steps:
+    as C3.
+C5  16 x 64^3 + 16 x 96^3 + 16 x 128^3 boxes, every other box multiphase,
+    each with its own seed, viscosity and drop count, stepped together
+    (batched, one captured graph per step): a sample of boxes of every size
+    and phase flag bit-equal to the same box stepped alone.
+
+With MHIT_CONFIG_REPORT set to a file name, each C2-C4 case appends its
+measured errors there as one JSON line.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _report(case, r):
+    print(case, json.dumps(r), flush=True)
+    path = os.environ.get("MHIT_CONFIG_REPORT")
+    if path:
+        with open(path, "a") as f:
+            f.write(json.dumps({"case": case, **r}) + "\n")
+
+
+def _compare(*argv):
+    from mhit36_mixedp_amd import precision
+    return precision.compare(precision.parser().parse_args(list(argv)))
+
+
+# ------------------------------------------------------------------ C1
+@pytest.mark.timeout(300)
+def test_c1_64_fp64_100_steps_against_oracle(oracle_mod):
+    from mhit36_mixedp_amd import Solver
+    print("[C1] 64^3 NS FP64, |k| <= 8, 100 steps vs the CPU oracle", flush=True)
+    n = 64
+    kw = dict(nx=n, poisson_precision=0, phiflag=0, partflag=0, dt=5e-4, mu=0.006, rho=1.0,
+              f1=1.0, f2=1.0, f3=1.0, k0=2.0)
+    with Solver(**kw) as s:
+        s.init_random_velocity(2024, kmax=8.0, urms=1.0)
+        o = oracle_mod.Oracle(**kw)
+        for f, a in zip((1, 2, 3), s.velocity()):
+            o.set_field(f, a)
+        worst = {"ke": 0.0, "dissipation": 0.0, "maxdiv": 0.0, "gumax": 0.0}
+        for step in range(100):
+            s.step(1)
+            assert o.step(1) == 0
+            ds, do = s.diag(), o.diag()
+            assert ds["step"] == do["step"] == step + 1
+            for k in worst:
+                worst[k] = max(worst[k], abs(ds[k] - do[k]) / abs(do[k]))
+        errs = {name: relerr(s.download_field(f), o.get_field(f))
+                for f, name in ((1, "u"), (2, "v"), (3, "w"), (4, "p"))}
+        o.close()
+    print(worst, errs, flush=True)
+    assert max(worst.values()) < 1e-10, worst
+    assert max(errs.values()) < 1e-10, errs
+
+
+# ------------------------------------------------------------ C2 - C4
+C2 = ("--n", "256", "--steps", "1000", "--dt", "2e-4", "--mu", "0.002")
+C3 = C2 + ("--phi", "1", "--ndrops", "64", "--radius", "0.2", "--ntracers", "1048576")
+C4 = ("--n", "512", "--steps", "1000", "--dt", "1e-4", "--mu", "0.002", "--phi", "1",
+      "--ndrops", "512", "--radius", "0.1", "--ntracers", "4194304")
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("k2", [0, 1], ids=["k2", "k2mod"])
+def test_c2_256_fp32_vs_fp64_1000_steps(k2):
+    print(f"[C2] 256^3 NS, 1000 steps, FP32 vs FP64 Poisson, k2_mode={k2}", flush=True)
+    r = _compare(*C2, "--k2", str(k2))
+    _report(f"C2 k2_mode={k2}", r)
+    assert r["nan_count"] == 0
+    assert r["ke_rel_err_max"] < 1e-6
+    assert r["dissipation_rel_err"] < 1e-6
+    if k2 == 1:
+        assert r["maxdiv_fp32"] < 1e-5 and r["maxdiv_fp64"] < 1e-10
+
+
+def _check_multiphase(r, k2):
+    assert r["nan_count"] == 0
+    assert r["ke_rel_err_max"] < 1e-6
+    assert r["dissipation_rel_err"] < 1e-6
+    # every ACDI term is a flux difference: phi mass is conserved to
+    # round-off in both forms, and both forms agree on it
+    assert r["phi_mass_drift_fp32"] < 1e-11 and r["phi_mass_drift_fp64"] < 1e-11
+    assert r["phi_mass_rel_err"] < 1e-11
+    # tracer positions of the FP32 form within 1e-4 of the FP64 form
+    # after 1000 steps (the grid spacing is 2.5e-2 at 256^3)
+    assert r["tracer_pos_err_max"] < 1e-4
+    if k2 == 1:
+        assert r["maxdiv_fp32"] < 1e-5 and r["maxdiv_fp64"] < 1e-10
+
+
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("k2", [0, 1], ids=["k2", "k2mod"])
+def test_c3_256_multiphase_tracers_1000_steps(k2):
+    print(f"[C3] 256^3, 64 drops, 1M tracers, 1000 steps, k2_mode={k2}", flush=True)
+    r = _compare(*C3, "--k2", str(k2))
+    _report(f"C3 k2_mode={k2}", r)
+    _check_multiphase(r, k2)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("k2", [0, 1], ids=["k2", "k2mod"])
+def test_c4_512_multiphase_tracers_1000_steps(k2):
+    print(f"[C4] 512^3, 512 drops, 4M tracers, 1000 steps, k2_mode={k2}", flush=True)
+    r = _compare(*C4, "--k2", str(k2))
+    _report(f"C4 k2_mode={k2}", r)
+    _check_multiphase(r, k2)
+
+
+# ------------------------------------------------------------------ C5
+@pytest.mark.timeout(300)
+def test_c5_ensemble_mix_equals_boxes_alone():
+    from mhit36_mixedp_amd import Ensemble, Solver, default_params
+    print("[C5] 16x64^3 + 16x96^3 + 16x128^3 batched vs boxes alone", flush=True)
+    sizes = (64, 96, 128)
+    params, seeds = [], []
+    for i in range(48):
+        params.append(default_params(nx=sizes[i // 16], phiflag=i % 2, partflag=0,
+                                     poisson_precision=1, dt=5e-4, mu=0.004 + 0.0005 * (i % 9),
+                                     rho=1.0, sigma=0.8, epsr=1.0))
+        seeds.append(2024 + i)
+
+    def init(b, i):
+        b.init_random_velocity(seeds[i], kmax=4.0, urms=1.0)
+        if i % 2:
+            b.init_drops(1 + i % 6, 0.4, seeds[i])
+
+    sample = [0, 1, 14, 15, 16, 17, 30, 31, 32, 33, 46, 47]
+    steps = 4
+    with Ensemble(params) as ens:
+        for i, b in enumerate(ens.boxes):
+            init(b, i)
+        ens.step(steps)
+        for i in sample:
+            with Solver(params[i]) as solo:
+                init(solo, i)
+                solo.step(steps)
+                a = ens.boxes[i]
+                fields = range(1, 10) if i % 2 else range(1, 9)
+                for f in fields:
+                    assert np.array_equal(a.download_field(f), solo.download_field(f)), (i, f)
+                assert a.diag() == solo.diag(), i
+        assert sum(b.diag()["nan_count"] for b in ens.boxes) == 0

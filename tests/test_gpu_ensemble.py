@@ -75,23 +75,33 @@ def test_ensemble_equals_solo(batched):
 @pytest.mark.parametrize("batched", [True, False])
 def test_ensemble_cfl_stop_isolated(batched):
     """A box stopped on cou > 7 stays frozen; the other boxes go on (in the
-    batched form both boxes share every launch)."""
-    from mhit36_mixedp_amd import CFLStop, Ensemble
+    batched form both boxes share every launch).  The frozen box goes
+    through three no-op steps inside the first call (an odd number of swaps
+    of the AB2 ping-pong buffers) and keeps the state, AB2 history
+    included, of the same box stopped alone."""
+    from mhit36_mixedp_amd import CFLStop, Ensemble, Solver
     print(f"[ensemble] CFL isolation, batched={batched}", flush=True)
     good = _params(dict(nx=16, phiflag=0, partflag=0))
-    bad = _params(dict(nx=16, phiflag=0, partflag=0))
+    bad = _params(dict(nx=16, phiflag=1, partflag=0, sigma=0.5))
     bad.dt = 0.5
-    with Ensemble([good, bad], batched=batched) as ens:
+    with Ensemble([good, bad], batched=batched) as ens, Solver(bad) as solo:
         ens.boxes[0].init_taylor_green()
-        ens.boxes[1].init_random_velocity(5, kmax=3.0, urms=20.0)
+        for s in (ens.boxes[1], solo):
+            s.init_random_velocity(5, kmax=3.0, urms=20.0)
+            s.init_drops(2, 0.6, 5)
         with pytest.raises(CFLStop):
-            ens.step(3)
+            ens.step(4)
+        with pytest.raises(CFLStop):
+            solo.step(1)
         assert ens.boxes[1].diag()["step"] == 1
-        assert ens.boxes[0].diag()["step"] == 3
+        assert ens.boxes[0].diag()["step"] == 4
         with pytest.raises(CFLStop):
             ens.step(2)
         assert ens.boxes[1].diag()["step"] == 1
-        assert ens.boxes[0].diag()["step"] == 5
+        assert ens.boxes[0].diag()["step"] == 6
+        for f in range(1, 10):
+            assert np.array_equal(ens.boxes[1].download_field(f), solo.download_field(f),
+                                  equal_nan=True), f
 
 
 def _ns_workgroups(boxes):

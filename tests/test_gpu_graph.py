@@ -67,7 +67,9 @@ def test_graph_cfl_stop(monkeypatch):
 @pytest.mark.parametrize("graph", ["1", "0"], ids=["graph", "eager"])
 def test_cfl_stop_mid_call_keeps_state(monkeypatch, graph):
     """Steps queued behind the CFL stop change nothing: the state (phi's
-    ping-pong buffers included) is the one after the stopping step."""
+    ping-pong buffers and the AB2 history, fields 6-9, included) is the one
+    after the stopping step.  step(4) queues three no-op steps behind the
+    stop, an odd number of swaps of the AB2 ping-pong buffers."""
     from mhit36_mixedp_amd import CFLStop, Solver
     monkeypatch.setenv("MHIT_GRAPH", graph)
     kw = dict(nx=16, dt=0.5, phiflag=1, sigma=0.5)
@@ -79,6 +81,20 @@ def test_cfl_stop_mid_call_keeps_state(monkeypatch, graph):
             a.step(4)
         with pytest.raises(CFLStop):
             b.step(1)
-        for f in (1, 2, 3, 4, 5):
+        for f in range(1, 10):
             assert np.array_equal(a.download_field(f), b.download_field(f), equal_nan=True), f
         assert a.diag()["step"] == 1
+        # a resumed run continues from the history of the last step taken
+        # (it may stop again: both runs then stop alike)
+        for s in (a, b):
+            s.set_ab2_state(1.5, 0.5, 1.0)
+            s.upload_field(1, np.zeros((16, 16, 16)))
+            s.upload_field(2, np.zeros((16, 16, 16)))
+            s.upload_field(3, np.zeros((16, 16, 16)))
+            try:
+                s.step(1)
+            except CFLStop:
+                pass
+        assert a.diag()["step"] == b.diag()["step"] == 2
+        for f in range(1, 10):
+            assert np.array_equal(a.download_field(f), b.download_field(f), equal_nan=True), f
