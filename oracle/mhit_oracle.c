@@ -859,13 +859,27 @@ int oracle_step(oracle_ctx* c, int nsteps) {
 /* ------------------------------------------------------------------ */
 /* Diagnostics (not in the solver; dissipation follows
  * post/compute_diss/main.f90:72-117 with dx = lx/n and nu = mu/rho).   */
+/* Compensated (Neumaier) running sum: the diagnostics sum up to 512^3
+ * terms of one sign, where a plain running sum loses up to ~1e-10 relative;
+ * this keeps the oracle's sums exact to a few ulp, so that they check the
+ * GPU's tree reductions rather than the other way round.                 */
+typedef struct { double s, c; } ksum;
+static inline void ksum_add(ksum* a, double x) {
+    double t = a->s + x;
+    if (fabs(a->s) >= fabs(x)) a->c += (a->s - t) + x;
+    else a->c += (x - t) + a->s;
+    a->s = t;
+}
+static inline double ksum_value(const ksum* a) { return a->s + a->c; }
+
 int oracle_diag(oracle_ctx* c, mhit_diag* d) {
     const int n = c->n;
     const double dxi = c->dxi, nu = c->P.mu / c->P.rho;
     memset(d, 0, sizeof(*d));
     d->step = c->step; d->alpha = c->alpha; d->beta = c->beta; d->gumax = c->gumax;
     d->cou = c->cou; d->umean = c->umean; d->vmean = c->vmean; d->wmean = c->wmean;
-    double ke = 0.0, diss = 0.0, maxdiv = 0.0, mass = 0.0;
+    ksum ke = {0.0, 0.0}, diss = {0.0, 0.0}, mass = {0.0, 0.0};
+    double maxdiv = 0.0;
     double pmin = 0.0, pmax = 0.0;
     int64_t nans = 0;
     for (int k = 0; k < n; k++) {
@@ -877,7 +891,7 @@ int oracle_diag(oracle_ctx* c, mhit_diag* d) {
                 size_t q = IDX(c, i, j, k);
                 double uu = c->u[q], vv = c->v[q], ww = c->w[q];
                 if (!isfinite(uu) || !isfinite(vv) || !isfinite(ww)) nans++;
-                ke += uu * uu + vv * vv + ww * ww;
+                ksum_add(&ke, uu * uu + vv * vv + ww * ww);
                 double dudx = (c->u[IDX(c, ip, j, k)] - c->u[IDX(c, im, j, k)]) * 0.5 * dxi;
                 double dudy = (c->u[IDX(c, i, jp, k)] - c->u[IDX(c, i, jm, k)]) * 0.5 * dxi;
                 double dudz = (c->u[IDX(c, i, j, kp)] - c->u[IDX(c, i, j, km)]) * 0.5 * dxi;
@@ -889,14 +903,14 @@ int oracle_diag(oracle_ctx* c, mhit_diag* d) {
                 double dwdz = (c->w[IDX(c, i, j, kp)] - c->w[IDX(c, i, j, km)]) * 0.5 * dxi;
                 double S11 = dudx, S22 = dvdy, S33 = dwdz;
                 double S12 = 0.5 * (dudy + dvdx), S13 = 0.5 * (dudz + dwdx), S23 = 0.5 * (dvdz + dwdy);
-                diss += S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+                ksum_add(&diss, S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23));
                 double dv = ((c->u[IDX(c, ip, j, k)] - uu) + (c->v[IDX(c, i, jp, k)] - vv) +
                              (c->w[IDX(c, i, j, kp)] - ww)) * dxi;
                 if (fabs(dv) > maxdiv) maxdiv = fabs(dv);
                 if (c->phi) {
                     double ph = c->phi[q];
                     if (!isfinite(ph)) nans++;
-                    mass += ph;
+                    ksum_add(&mass, ph);
                     if (q == 0 || ph < pmin) pmin = ph;
                     if (q == 0 || ph > pmax) pmax = ph;
                 }
@@ -904,10 +918,10 @@ int oracle_diag(oracle_ctx* c, mhit_diag* d) {
         }
     }
     double N3 = (double)c->N3;
-    d->ke = 0.5 * ke / N3;
-    d->dissipation = 2.0 * nu * diss / N3;
+    d->ke = 0.5 * ksum_value(&ke) / N3;
+    d->dissipation = 2.0 * nu * ksum_value(&diss) / N3;
     d->maxdiv = maxdiv;
-    d->phi_mass = mass * c->dx * c->dx * c->dx;
+    d->phi_mass = ksum_value(&mass) * c->dx * c->dx * c->dx;
     d->phi_min = pmin; d->phi_max = pmax;
     d->nan_count = nans;
     return MHIT_OK;

@@ -160,7 +160,8 @@ def test_c5_ensemble_mix_equals_boxes_alone():
                 init(solo, i)
                 solo.step(steps)
                 a = ens.boxes[i]
-                fields = range(1, 10) if i % 2 else range(1, 9)
+                # u, v, w, p, rhs{u,v,w}_o; phi and rhsphi_o for multiphase boxes
+                fields = (1, 2, 3, 4, 5, 6, 7, 8, 9) if i % 2 else (1, 2, 3, 4, 6, 7, 8)
                 for f in fields:
                     assert np.array_equal(a.download_field(f), solo.download_field(f)), (i, f)
                 assert a.diag() == solo.diag(), i

@@ -7,7 +7,13 @@ for 1 step.
 Bars: FP64 Poisson 1e-12 relative to each field's largest value, FP32
 Poisson 1e-6 against the oracle's FP32 form; velocity, pressure, phi,
 tracer positions and velocities (matched by tag) and the diagnostic
-scalars.  The oracle takes seconds per 512^3 step on the host's cores."""
+scalars.  One FP32 exception: max |div u| after the projection is the
+difference of two nearly equal fields, div u* and dt/rho lap_h p, and the
+FP32 transforms' rounding of p (the two FFTs differ in algorithm, about
+4e-7 relative in p) is amplified by lap_h toward the grid scale; its bar
+in FP32 is 1e-5 (measured 4.2e-6 at 256^3).  The oracle's sums are
+compensated, so the diagnostic sums are checked against near-exact values.
+The oracle takes seconds per 512^3 step on the host's cores."""
 import pytest
 
 from test_gpu_step import compare, make_pair
@@ -36,6 +42,8 @@ def _run(oracle_mod, n, prec, steps, ndrops, radius, ntr, dt):
         o.close()
     print(e, flush=True)
     tol = 1e-12 if prec == 0 else 1e-6
+    if prec == 1:
+        assert e.pop("diag_maxdiv") < 1e-5, e
     assert max(e.values()) < tol, e
 
 
