@@ -233,6 +233,19 @@ int mhit_drop_stats(mhit_ctx* ctx, double threshold, int connectivity,
  * zero-filled up to nbins.                                                */
 int mhit_energy_spectrum(mhit_ctx* ctx, int32_t nbins, double* E, int32_t* nshells);
 
+/* mhit_energy_spectrum without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The shell sums go to a fixed per-device array of
+ * MHIT_SPECTRUM_NOALLOC_MAX_BINS doubles that the library holds until the
+ * process ends; min(nbins, shells) above that limit returns MHIT_ERR_ARG.
+ * Every call on a device shares its array: calls that overlap on one
+ * device (from several threads or contexts) race.  Results as
+ * mhit_energy_spectrum's (the shell sums are atomic adds, so the two agree
+ * to rounding, not bit for bit).                                          */
+#define MHIT_SPECTRUM_NOALLOC_MAX_BINS 4096
+int mhit_energy_spectrum_noalloc(mhit_ctx* ctx, int32_t nbins, double* E, int32_t* nshells);
+
 /* Block until all work queued on the context's stream is done.            */
 int mhit_sync(mhit_ctx* ctx);
 

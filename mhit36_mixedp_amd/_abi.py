@@ -193,6 +193,7 @@ EXPORTS = [
     ("mhit_set_tuning", ctypes.c_int, [_vp, _P(Tuning)]),
     ("mhit_autotune", ctypes.c_int, [_vp, ctypes.c_int, _P(Tuning)]),
     ("mhit_energy_spectrum", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _P(ctypes.c_int32)]),
+    ("mhit_energy_spectrum_noalloc", ctypes.c_int, [_vp, ctypes.c_int32, _dp, _P(ctypes.c_int32)]),
     ("mhit_drop_stats", ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int64, _P(Drop),
                                        _P(ctypes.c_int64)]),
     ("mhit_last_error", ctypes.c_char_p, []),

@@ -209,13 +209,15 @@ class Solver:
         _check(self.lib.mhit_autotune(self._h, int(steps), ctypes.byref(t)), "mhit_autotune")
         return t.as_dict()
 
-    def energy_spectrum(self) -> np.ndarray:
-        """E(k), k = 0, 1, ...: shell spectrum of the kinetic energy."""
+    def energy_spectrum(self, noalloc: bool = False) -> np.ndarray:
+        """E(k), k = 0, 1, ...: shell spectrum of the kinetic energy.
+        noalloc=True uses mhit_energy_spectrum_noalloc (no device
+        allocation; shell sums in the library's fixed per-device array)."""
         nb = int(np.floor(np.sqrt(3.0) * (self.n // 2) + 0.5)) + 1
         E = np.zeros(nb)
         ns = ctypes.c_int32()
-        _check(self.lib.mhit_energy_spectrum(self._h, nb, _ptr(E), ctypes.byref(ns)),
-               "mhit_energy_spectrum")
+        fn = "mhit_energy_spectrum_noalloc" if noalloc else "mhit_energy_spectrum"
+        _check(getattr(self.lib, fn)(self._h, nb, _ptr(E), ctypes.byref(ns)), fn)
         return E[:ns.value]
 
     def drop_stats(self, threshold: float = 0.5, connectivity: int = 6, max_drops: int = 1 << 16):

@@ -14,7 +14,8 @@
 #include "internal.hpp"
 
 #include <cmath>
-#include <vector>
+#include <mutex>
+#include <string>
 
 namespace mhit {
 int sync_scalars_public(mhit_ctx* c);
@@ -50,22 +51,25 @@ __global__ void k_shell_bins(const V* __restrict__ spec, const V* __restrict__ n
         if (sb[b] != 0.0) atomicAdd(&bins[b], sb[b]);
 }
 
-}  // namespace
+// bins[0, nb) = 0 (a fill kernel rather than a memset node, so the same
+// launch sequence can sit in a captured graph)
+__global__ void k_zero_bins(double* __restrict__ bins, int nb) {
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += gridDim.x * blockDim.x) bins[b] = 0.0;
+}
 
-extern "C" {
+// Number of shells the transform of an n^3 box fills.
+int shells_used(int n) { return (int)std::floor(std::sqrt(3.0) * (n / 2) + 0.5) + 1; }
 
-int mhit_energy_spectrum(mhit_ctx* c, int32_t nbins, double* E, int32_t* nshells) {
-    if (!c || nbins <= 0 || !E) { set_error("bad arguments"); return MHIT_ERR_ARG; }
-    int rc;
-    if ((rc = sync_scalars_public(c))) return rc;
-    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+// The spectrum with its shell sums in bins[0, min(nbins, shells)) (device
+// memory the caller provides); E is written directly, no host staging.
+int spectrum_into(mhit_ctx* c, int32_t nbins, double* E, int32_t* nshells, double* bins) {
     const int n = c->n;
-    const int used = (int)std::floor(std::sqrt(3.0) * (n / 2) + 0.5) + 1;
+    const int used = shells_used(n);
     if (nshells) *nshells = used;
     const int nb = nbins < used ? nbins : used;
-    double* bins = nullptr;
-    MHIT_HIP_TRY(hipMalloc((void**)&bins, (size_t)nb * sizeof(double)));
-    hipError_t err = hipMemsetAsync(bins, 0, (size_t)nb * sizeof(double), c->stream);
+    int rc = MHIT_OK;
+    k_zero_bins<<<div_up(nb, 256), 256, 0, c->stream>>>(bins, nb);
+    hipError_t err = hipGetLastError();
     const double N = (double)n * n * n;
     const double scale = 0.5 / (N * N);
     double e0 = 0.0;   // the k = 0 shell, from the true mean
@@ -97,16 +101,85 @@ int mhit_energy_spectrum(mhit_ctx* c, int32_t nbins, double* E, int32_t* nshells
         const double m = f0[0] / N - c->S_host.mean[f];
         e0 += 0.5 * m * m;
     }
-    std::vector<double> h(nb);
     if (err == hipSuccess && rc == MHIT_OK)
-        err = hipMemcpyAsync(h.data(), bins, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        err = hipMemcpyAsync(E, bins, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (err == hipSuccess && rc == MHIT_OK) err = hipStreamSynchronize(c->stream);
-    (void)hipFree(bins);
     if (rc) return rc;
     MHIT_HIP_TRY(err);
-    h[0] = e0;
-    for (int b = 0; b < nbins; b++) E[b] = b < nb ? h[b] : 0.0;
+    E[0] = e0;
+    for (int b = nb; b < nbins; b++) E[b] = 0.0;
     return MHIT_OK;
+}
+
+int spectrum_check(mhit_ctx* c, int32_t nbins, double* E) {
+    if (!c || nbins <= 0 || !E) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    int rc;
+    if ((rc = sync_scalars_public(c))) return rc;
+    if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+    return MHIT_OK;
+}
+
+// Shell sums of mhit_energy_spectrum_noalloc: one array per device, held
+// until the process ends, shared by every call on that device (calls that
+// overlap on one device race on it).  Host code never uses the array's
+// name as an address: each device's copy is looked up once with
+// hipGetSymbolAddress (with that device current) and kept.
+constexpr int kScratchBins = MHIT_SPECTRUM_NOALLOC_MAX_BINS;
+static __device__ __align__(256) double scratch[kScratchBins];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch[kMaxDevices] = {};
+
+int scratch_on(int device, double** out) {
+    if (device < 0 || device >= kMaxDevices) { set_error("device ordinal out of range"); return MHIT_ERR_ARG; }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[device]) {
+        int cur = -1;
+        MHIT_HIP_TRY(hipGetDevice(&cur));
+        if (cur != device) MHIT_HIP_TRY(hipSetDevice(device));
+        double* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+        if (cur != device) MHIT_HIP_TRY(hipSetDevice(cur));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            return MHIT_ERR_HIP;
+        }
+        g_scratch[device] = p;
+    }
+    *out = g_scratch[device];
+    return MHIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhit_energy_spectrum(mhit_ctx* c, int32_t nbins, double* E, int32_t* nshells) {
+    int rc;
+    if ((rc = spectrum_check(c, nbins, E))) return rc;
+    const int used = shells_used(c->n);
+    const int nb = nbins < used ? nbins : used;
+    double* bins = nullptr;
+    MHIT_HIP_TRY(hipMalloc((void**)&bins, (size_t)nb * sizeof(double)));
+    rc = spectrum_into(c, nbins, E, nshells, bins);
+    (void)hipFree(bins);
+    return rc;
+}
+
+int mhit_energy_spectrum_noalloc(mhit_ctx* c, int32_t nbins, double* E, int32_t* nshells) {
+    int rc;
+    if ((rc = spectrum_check(c, nbins, E))) return rc;
+    const int used = shells_used(c->n);
+    const int nb = nbins < used ? nbins : used;
+    if (nb > kScratchBins) {
+        set_error("mhit_energy_spectrum_noalloc: " + std::to_string(nb) + " shells exceed the " +
+                  std::to_string(kScratchBins) + "-shell scratch (ask for fewer bins)");
+        return MHIT_ERR_ARG;
+    }
+    double* bins = nullptr;
+    if ((rc = scratch_on(c->device, &bins))) return rc;
+    return spectrum_into(c, nbins, E, nshells, bins);
 }
 
 }  // extern "C"
