@@ -196,13 +196,22 @@ typedef struct mhit_tuning {
     int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
     int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8        */
     int32_t ac_my;        /* tile rows of the fused ACDI pass: 16 or 8      */
+    int32_t fft_wave;     /* 1: column transforms of eight columns (the z
+                             pass; the y pass in FP64) in the wavefront form,
+                             lane/register transposes by permlane and DPP
+                             (n = 64, 128, 256, 512); 0: the register-direct
+                             form with LDS exchanges.  The two forms compute
+                             the same DFT in a different operation order, so
+                             they differ in rounding only; the other fields
+                             change no result bit.                          */
     double ms_per_step;   /* mhit_autotune: measured with these settings    */
 } mhit_tuning;
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);
 int mhit_set_tuning(mhit_ctx* ctx, const mhit_tuning* t);
-/* Time candidate shapes on the context's own state (`steps` timed steps
- * per candidate), keep the fastest, and restore the state: the run then
- * continues exactly as if the call had not been made.                     */
+/* Time candidate shapes and transform forms on the context's own state
+ * (`steps` timed steps per candidate), keep the fastest, and restore the
+ * state: the run then continues exactly as if the call had not been made
+ * (apart from the rounding of the transform form it picks).               */
 int mhit_autotune(mhit_ctx* ctx, int steps, mhit_tuning* out);
 
 /* Drop statistics from the phase field (post-processing of the reference,

@@ -92,7 +92,8 @@ class Diag(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """mhit_tuning: launch shapes (none changes a result bit)."""
+    """mhit_tuning: launch shapes (no result bit changes) and the column
+    transforms' form (fft_wave: rounding-level differences only)."""
 
     _fields_ = [
         ("ns_mz", ctypes.c_int32),
@@ -101,6 +102,7 @@ class Tuning(ctypes.Structure):
         ("sort_every", ctypes.c_int32),
         ("ns_my", ctypes.c_int32),
         ("ac_my", ctypes.c_int32),
+        ("fft_wave", ctypes.c_int32),
         ("ms_per_step", ctypes.c_double),
     ]
 

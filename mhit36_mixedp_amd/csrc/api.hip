@@ -91,6 +91,7 @@ static void parse_tuning(const char* s, Tuning& t) {
             else if (key == "ac_my" && (val == 8 || val == 16)) t.ac_my = val;
             else if (key == "xcd") t.xcd = val != 0;
             else if (key == "sort" && val >= 0) t.sort_every = val;
+            else if (key == "fft") t.fft_wave = val != 0;
         }
         pos = end + 1;
     }
@@ -329,6 +330,14 @@ int step_async(mhit_ctx* c, int nsteps) {
 int finish_steps(mhit_ctx* c) {
     int rc;
     if ((rc = sync_scalars(c))) return rc;
+    if (c->S_host.skipped) {
+        // steps queued behind a CFL stop wrote no AB2 history but the host
+        // still swapped the history buffers after each: an odd count leaves
+        // the last history in the "new" buffer, so swap once more
+        if (c->march && (c->S_host.skipped & 1)) ns_swap(c);
+        c->S_host.skipped = 0;
+        if ((rc = push_scalars(c))) return rc;
+    }
     if (c->S_host.halted) {
         set_error("cou = " + std::to_string(c->S_host.cou) + " > 7 at step " +
                   std::to_string(c->S_host.step) + " (main.f90:1006)");

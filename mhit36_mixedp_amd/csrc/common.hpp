@@ -36,13 +36,15 @@ struct DevScalars {
                           // velocity is u_stored = u + mean (subtracted on load)
     long long step;       // completed steps
     int halted;           // set when cou > 7 (cfl_abort); kernels then no-op
-    int pad_;
+    int skipped;          // steps run while halted since the host last looked
 };
 
-// Launch-shape choices that do not change results (z-chunk lengths and
-// tile rows of the marching passes, XCD-aware tile order).  Defaults are
-// set by grid size at create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,
-// ac_my=8,xcd=1,sort=10" overrides them.
+// Launch-shape choices (z-chunk lengths and tile rows of the marching
+// passes, XCD-aware tile order), which do not change results, and the form
+// of the column transforms, which changes them only by rounding (the same
+// DFT with a different operation order).  Defaults are set by grid size at
+// create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,ac_my=8,xcd=1,sort=10,fft=1"
+// overrides them.
 struct Tuning {
     int ns_mz = 64;       // planes per workgroup, NS pass (set by grid size at create)
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
@@ -50,6 +52,9 @@ struct Tuning {
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
     int ns_my = 16;       // tile rows (16 or 8), NS pass (mhit_create: 8 for n <= 64)
     int ac_my = 16;       // tile rows (16 or 8), ACDI pass (mhit_create: 8 for n <= 64)
+    int fft_wave = 1;     // column transforms of eight columns in the wavefront
+                          // form (lane/register transposes by permlane and DPP)
+                          // where n = 64, 128, 256, 512; 0: register-direct form
 };
 
 // Per-run constants passed by value to kernels.
@@ -134,6 +139,39 @@ __device__ __forceinline__ void gstore(MHIT_GLOBAL V* p, const V& v) {
     t.x = v.x;
     t.y = v.y;
     *(MHIT_GLOBAL E*)p = t;
+}
+
+// Accesses of data that a pass touches exactly once (its outputs, the
+// correction's and the inverse x transform's inputs): the nontemporal hint
+// (`nt`) streams them through the caches instead of letting them displace
+// lines that neighbouring workgroups read again (halo rows and columns).
+// Measured at 512^3: nontemporal stores in every pass 8.56-8.71 -> 8.34 ms
+// per step; nontemporal loads only where they paid (the NS, ACDI and y
+// passes read their once-used inputs faster with plain loads).
+template <typename T>
+__device__ __forceinline__ T ld_once(const MHIT_GLOBAL T* p) {
+    return __builtin_nontemporal_load(p);
+}
+template <typename T>
+__device__ __forceinline__ void st_once(MHIT_GLOBAL T* p, T v) {
+    __builtin_nontemporal_store(v, p);
+}
+template <typename V>
+__device__ __forceinline__ V gload_once(const MHIT_GLOBAL V* p) {
+    typedef typename V::value_type E __attribute__((ext_vector_type(2)));
+    const E t = ld_once((const MHIT_GLOBAL E*)p);
+    V r;
+    r.x = t.x;
+    r.y = t.y;
+    return r;
+}
+template <typename V>
+__device__ __forceinline__ void gstore_once(MHIT_GLOBAL V* p, const V& v) {
+    typedef typename V::value_type E __attribute__((ext_vector_type(2)));
+    E t;
+    t.x = v.x;
+    t.y = v.y;
+    st_once((MHIT_GLOBAL E*)p, t);
 }
 
 __device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz, const Blk& b, int enable = 1) {

@@ -144,7 +144,7 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
     };
     for (auto& gp : group_by(boxes, ns_key))
         if ((rc = launch_ns_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
-    auto prec_key = [](mhit_ctx* c) { return c->fp32; };
+    auto prec_key = [](mhit_ctx* c) { return std::make_pair(c->fp32, c->tune.fft_wave); };
     const auto by_prec = group_by(boxes, prec_key);
     for (auto& gp : by_prec)
         if ((rc = poisson_project_batch(gp.data(), (int)gp.size(), ar, st))) return rc;

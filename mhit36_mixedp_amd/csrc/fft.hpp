@@ -31,6 +31,11 @@ template <typename V> __device__ __forceinline__ V c_sub(V a, V b) { V r; r.x = 
 template <typename V> __device__ __forceinline__ V c_mul(V a, V b) {
     V r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
 }
+// a b with two fused multiply-adds per component product pair (the
+// wavefront-form transforms' twiddles): 4 operations instead of 6
+template <typename V> __device__ __forceinline__ V c_mul_fma(V a, V b) {
+    V r; r.x = fma(a.x, b.x, -(a.y * b.y)); r.y = fma(a.x, b.y, a.y * b.x); return r;
+}
 // multiply by -i (forward) or +i (inverse)
 template <bool INV, typename V> __device__ __forceinline__ V c_rot(V a) {
     V r;
@@ -196,6 +201,57 @@ __device__ __forceinline__ void fft_last8_from_lds(const V* lds, int c, int t, V
     dft8<INV>(v);
 }
 
+// ------------------------------------------------ wavefront exchanges
+// Transpose of the lane digit b = lane bits 3-5 with the register digit r
+// (index 0..7) of eight values per lane, inside the wavefront: afterwards
+// lane (b, l) holds in v[r] what lane (r, l) held in v[b] (l = lane bits
+// 0-2).  Three bit swaps: lane bit 5 with index bit 2 by
+// v_permlane32_swap (the upper 32 lanes of the first operand trade places
+// with the lower 32 of the second), lane bit 4 with index bit 1 by
+// v_permlane16_swap (the same on 16-lane rows), lane bit 3 with index bit
+// 0 by a DPP row rotation of 8 lanes (lane ^ 8 within each 16-lane row)
+// and a select.  No LDS, no barrier.
+template <typename V>
+__device__ __forceinline__ void xpose_lane_hi(V (&v)[8]) {
+    constexpr int DW = sizeof(V) / 4;
+    unsigned d[8][DW];
+#pragma unroll
+    for (int r = 0; r < 8; r++) __builtin_memcpy(d[r], &v[r], sizeof(V));
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < DW; q++) {
+            const auto p = __builtin_amdgcn_permlane32_swap(d[r][q], d[r + 4][q], false, false);
+            d[r][q] = p[0];
+            d[r + 4][q] = p[1];
+        }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        if (r & 2) continue;
+#pragma unroll
+        for (int q = 0; q < DW; q++) {
+            const auto p = __builtin_amdgcn_permlane16_swap(d[r][q], d[r + 2][q], false, false);
+            d[r][q] = p[0];
+            d[r + 2][q] = p[1];
+        }
+    }
+    const bool odd = (__lane_id() >> 3) & 1;
+#pragma unroll
+    for (int r = 0; r < 8; r += 2)
+#pragma unroll
+        for (int q = 0; q < DW; q++) {
+            // lanes with bit 3 clear send their odd value and take the
+            // partner's even one into it; lanes with bit 3 set the reverse
+            const unsigned s = odd ? d[r][q] : d[r + 1][q];
+            const unsigned x = (unsigned)__builtin_amdgcn_update_dpp(0, (int)s, 0x128 /* row_ror:8 */,
+                                                                     0xF, 0xF, false);
+            if (odd) d[r][q] = x;
+            else d[r + 1][q] = x;
+        }
+#pragma unroll
+    for (int r = 0; r < 8; r++) __builtin_memcpy(&v[r], d[r], sizeof(V));
+}
+
 // Copy the twiddle table W_N^j, j < N, into LDS.  Stages read their
 // twiddles from there: the C threads of one butterfly row share an address
 // (broadcast), and no stage waits on a global load.
@@ -226,5 +282,13 @@ template <typename T, int N> struct FftCfg {
     // threads per column
     static constexpr int TPC = (N % 8 == 0) ? N / 8 : ((N % 4 == 0) ? N / 4 : N / 2);
 };
+
+// Whether a column pass of C columns can take the wavefront form (eight
+// strided columns, N = 64 M with M = 1, 2, 4 or 8, N/8 threads per column).
+template <typename T, int N, int C, bool AXIS_CONTIG>
+constexpr bool wave_form_fits() {
+    return !AXIS_CONTIG && C == 8 && N % 64 == 0 && FftCfg<T, N>::TPC == N / 8 &&
+           (N == 64 || N == 128 || N == 256 || N == 512);
+}
 
 }  // namespace mhit

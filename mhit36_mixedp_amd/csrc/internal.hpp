@@ -236,6 +236,8 @@ int launch_stream_probe(mhit_ctx* c, const double* const* in, int R, double* con
 
 // march.hip
 int launch_acdi_march(mhit_ctx* c);
+// Swap the roles of the two AB2 history buffers of the NS pass.
+void ns_swap(mhit_ctx* c);
 int launch_ns_march(mhit_ctx* c);
 // Batched forms: boxes[0..nb) share the size, flags and launch shapes; one
 // launch on stream st, argument table in the arena; the role swaps of every

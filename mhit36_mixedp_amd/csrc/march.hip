@@ -111,19 +111,11 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const int z1 = min(z0 + zc, n);
     const int i = i0 + tx, j = j0 + ty;
     const bool live = i < n && j < n;
-    if (S->halted) {
-        // a stopped run keeps its state: the new AB2 history becomes a copy
-        // of the old one, so the host-side swap of the two buffers leaves
-        // the history of the last step taken in place
-        if (live)
-            for (int k = z0; k < z1; k++) {
-                const size_t q = idx3(i, j, k, n);
-                ru_n[q] = ru_o[q];
-                rv_n[q] = rv_o[q];
-                rw_n[q] = rw_o[q];
-            }
-        return;
-    }
+    // A stopped run writes nothing; its AB2 history stays in the buffer it
+    // was in, and the host undoes an odd number of buffer swaps afterwards
+    // (the finalize pass counts the skipped steps).  No copy here: any code
+    // in this branch costs the marching loop registers.
+    if (S->halted) return;
     // global indices of the cell, or of its periodic image past the domain edge
     const int ic = wrapn(i, n), jc = wrapn(j, n);
     const double m0 = S->mean[0], m1 = S->mean[1], m2 = S->mean[2];
@@ -310,17 +302,17 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
         if (live && k > z0) {
             // main.f90:841-844: rhsp = rho/(dt dx) sum_d (u*_d(+1) - u*_d)
             const double r = dprev + cdiv * (wsk - wsprev);
-            div[idx3(i, j, k - 1, n)] = (TD)r;
+            st_once(&div[idx3(i, j, k - 1, n)], (TD)r);
         }
         double usk = 0.0, vsk = 0.0;
         if (!last) {
-            if (live) { ws[q] = wsk; rw_n[q] = rw; }
+            if (live) { st_once(&ws[q], wsk); st_once(&rw_n[q], rw); }
             const double ru = rhs_u(sm, sc, sq, tx, ty, jc, k);
             usk = su[sc][c] + dt * (alpha * ru - beta * rou);
-            if (live) { us[q] = usk; ru_n[q] = ru; }
+            if (live) { st_once(&us[q], usk); st_once(&ru_n[q], ru); }
             const double rv = rhs_v(sm, sc, sq, tx, ty, ic, k);
             vsk = sv[sc][c] + dt * (alpha * rv - beta * rov);
-            if (live) { vs[q] = vsk; rv_n[q] = rv; }
+            if (live) { st_once(&vs[q], vsk); st_once(&rv_n[q], rv); }
             sus[ty * (MX + 1) + tx] = usk;
             svs[ty * MX + tx] = vsk;
             if (xedge) {
@@ -540,8 +532,8 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
                              (Fy[(ty + 2) * MX + tx] - Fy[ty * MX + tx]) * 0.5 * dxi +
                              (fz_p1 - fz_m1) * 0.5 * dxi);
             // P5 AB2, main.f90:625-626
-            phi_out[q] = pc + dt * (alpha * r - beta * ro);
-            rphi_o[q] = r;
+            st_once(&phi_out[q], pc + dt * (alpha * r - beta * ro));
+            st_once(&rphi_o[q], r);
             wk = wp;
         }
         fz_m1 = fz_0;
@@ -585,7 +577,7 @@ static AcdiArgs acdi_args(mhit_ctx* c) {
 
 // the new AB2 history is in the other buffers (neighbours' old values are
 // read during the pass)
-static void ns_swap(mhit_ctx* c) {
+void ns_swap(mhit_ctx* c) {
     std::swap(c->rhsu_o, c->rhsu_n);
     std::swap(c->rhsv_o, c->rhsv_n);
     std::swap(c->rhsw_o, c->rhsw_n);

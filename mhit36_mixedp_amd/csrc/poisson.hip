@@ -83,8 +83,8 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
         const int a = e % N, c = e / N;
         V val;
         if (IN == X_IN_READY) {
-            val.x = din[idx3(a, y0 + 2 * c, z, N)];
-            val.y = din[idx3(a, y0 + 2 * c + 1, z, N)];
+            val.x = ld_once(&din[idx3(a, y0 + 2 * c, z, N)]);
+            val.y = ld_once(&din[idx3(a, y0 + 2 * c + 1, z, N)]);
             lds[a * LD + c] = val;
             continue;
         }
@@ -127,14 +127,14 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
             out.x = (zk.y + zm.y) * (T)0.5;
             out.y = (zm.x - zk.x) * (T)0.5;
         }
-        gstore(spec + ((size_t)z * N + y0 + rr) * H + k, out);
+        gstore_once(spec + ((size_t)z * N + y0 + rr) * H + k, out);
     }
     for (int rr = tid; rr < 2 * P; rr += NT) {
         const V zh = lds[H * LD + (rr >> 1)];
         V out;
         out.x = (rr & 1) ? zh.y : zh.x;
         out.y = (T)0;
-        gstore(nyq + (size_t)z * N + y0 + rr, out);
+        gstore_once(nyq + (size_t)z * N + y0 + rr, out);
     }
 }
 
@@ -178,12 +178,12 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
         if ((H * P) % NT != 0 && e >= H * P) break;
         const int k = e % H, c = e / H;
         const size_t ra = ((size_t)z * N + y0 + 2 * c) * H;
-        const V A = gload(spec + ra + k);
-        const V B = gload(spec + ra + H + k);
+        const V A = gload_once(spec + ra + k);
+        const V B = gload_once(spec + ra + H + k);
         if (k == 0) {
             V z0; z0.x = A.x; z0.y = B.x;
-            const V An = gload(nyq + (size_t)z * N + y0 + 2 * c);
-            const V Bn = gload(nyq + (size_t)z * N + y0 + 2 * c + 1);
+            const V An = gload_once(nyq + (size_t)z * N + y0 + 2 * c);
+            const V Bn = gload_once(nyq + (size_t)z * N + y0 + 2 * c + 1);
             V zh; zh.x = An.x; zh.y = Bn.x;
             lds[c] = z0;
             lds[H * LD + c] = zh;
@@ -203,7 +203,7 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
         if ((N * 2 * P) % NT != 0 && e >= N * 2 * P) break;
         const int x = e % N, rr = e / N;
         const V zz = lds[x * LD + (rr >> 1)];
-        pres[idx3(x, y0 + rr, z, N)] = (rr & 1) ? zz.y : zz.x;
+        st_once(&pres[idx3(x, y0 + rr, z, N)], (rr & 1) ? zz.y : zz.x);
     }
 }
 
@@ -223,7 +223,7 @@ struct ColArgs {
     double ntot;
     const DevScalars* S;
 };
-template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool NYQ>
+template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool NYQ, bool WF>
 __global__ __launch_bounds__((C * FftCfg<T, N>::TPC))
 void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
@@ -258,8 +258,147 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
         byi = t / blk.nx;
     }
     MHIT_GLOBAL V* __restrict__ g = data + bxi * bx + byi * by;
+    // whole 128-byte lines per row segment: stream the stores; a half-line
+    // segment's other half is written by the paired workgroup
+    constexpr bool LINES = C * (int)sizeof(V) >= 128;
+    auto ld = [](const MHIT_GLOBAL V* p) -> V { return gload(p); };
+    auto st = [](MHIT_GLOBAL V* p, const V& v) {
+        if constexpr (LINES) gstore_once(p, v);
+        else gstore(p, v);
+    };
 
     static_assert((N * C) % NT == 0, "column tile must be a multiple of the workgroup");
+    // Wavefront form: N = 8 M 8 (M = 1, 2, 4, 8), eight columns, N/8
+    // threads per column, thread t = b + 8 w of column c in lane 8 b + c of
+    // wave w.  With n = n0 + 8 n1 + 8M n2 the thread holds x[t + 8M r], so
+    // n0 = b is a lane digit, n1 = w the wave and n2 = r the register.
+    // Decimation in frequency, k = k0 + 8 k1 + 8M k2:
+    //   radix 8 over n2 in registers -> k0; twiddle W_N^(k0 t);
+    //   LDS exchange (the only cross-wave step): wave <- k0 (8/M values
+    //   per wave), registers <- n1;
+    //   radix M over n1 -> k1; twiddle W_N^(8 n0 k1);
+    //   lane/register transpose inside the wave (xpose_lane_hi) -> n0;
+    //   radix 8 over n0 -> k2.
+    // X[k] is then in wave w, lane digit b, register r with
+    // k = spec_k(w, b, r) below.  The inverse runs the same steps backwards
+    // from that layout and ends in the natural one, so the z pass needs
+    // two LDS exchanges (three barriers with the twiddle staging) instead of
+    // four exchanges and seven barriers.  N = 64 has no cross-wave step.
+    // Each wave writes its half of the inverse exchange to the same 512
+    // values it read in the forward one, so no barrier separates them.
+    constexpr bool WAVE = WF && wave_form_fits<T, N, C, AXIS_CONTIG>();
+    if constexpr (WAVE) {
+        constexpr int M = N / 64, S = 8 / M;
+        const int c = tid & 7, b = (tid >> 3) & 7, w = tid >> 6, t = b + 8 * w;
+        V v[8];
+        auto spec_k = [&](int r) { return (w * S + b / M) + 8 * (b % M) + 8 * M * r; };
+        if constexpr (MODE == 1) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) v[r] = ld(g + (size_t)spec_k(r) * sa + c * sc);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) v[r] = ld(g + (size_t)(t + r * TPC) * sa + c * sc);
+        }
+        stage_twiddles<V, N, NT>(stw, tw, tid);
+        __shared__ double skk[MODE == 2 ? N : 1];
+        if constexpr (MODE == 2) {
+            for (int e = tid; e < N; e += NT) skk[e] = kk[e];
+        }
+        lds_sync();
+        auto twiddle = [&](V& x, int j, auto inv) {
+            V wj = stw[j];
+            if (decltype(inv)::value) wj.y = -wj.y;
+            x = c_mul_fma(x, wj);
+        };
+        auto forward = [&]() {
+            dft8<false>(v);
+#pragma unroll
+            for (int r = 1; r < 8; r++) twiddle(v[r], r * t, std::false_type{});
+            if constexpr (M > 1) {
+#pragma unroll
+                for (int r = 0; r < 8; r++) lds[(r * TPC + t) * LD + c] = v[r];
+                lds_sync();
+#pragma unroll
+                for (int s = 0; s < S; s++)
+#pragma unroll
+                    for (int q = 0; q < M; q++) v[s * M + q] = lds[((w * S + s) * TPC + q * 8 + b) * LD + c];
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    dft<M, false>(v + s * M);
+#pragma unroll
+                    for (int q = 1; q < M; q++) twiddle(v[s * M + q], 8 * b * q, std::false_type{});
+                }
+            }
+            xpose_lane_hi(v);
+            dft8<false>(v);
+        };
+        // the inverse from the layout the forward transform leaves; b is
+        // then the digit s M + k1 and, after the transpose, n0
+        auto inverse = [&]() {
+            dft8<true>(v);
+            if constexpr (M > 1) {
+#pragma unroll
+                for (int r = 1; r < 8; r++) twiddle(v[r], 8 * (b % M) * r, std::true_type{});
+            }
+            xpose_lane_hi(v);
+            if constexpr (M > 1) {
+#pragma unroll
+                for (int s = 0; s < S; s++) {
+                    dft<M, true>(v + s * M);
+#pragma unroll
+                    for (int q = 0; q < M; q++) {
+                        const int j = (w * S + s) * (b + 8 * q);
+                        if (j) twiddle(v[s * M + q], j, std::true_type{});
+                    }
+                }
+#pragma unroll
+                for (int s = 0; s < S; s++)
+#pragma unroll
+                    for (int q = 0; q < M; q++) lds[((w * S + s) * TPC + q * 8 + b) * LD + c] = v[s * M + q];
+                lds_sync();
+#pragma unroll
+                for (int r = 0; r < 8; r++) v[r] = lds[(r * TPC + t) * LD + c];
+            } else {
+#pragma unroll
+                for (int r = 1; r < 8; r++) twiddle(v[r], r * b, std::true_type{});
+            }
+            dft8<true>(v);
+        };
+        if constexpr (MODE == 1) {
+            inverse();
+        } else {
+            forward();
+        }
+        if constexpr (MODE == 2) {
+            // main.f90:883-896: p^ = -r^/(k2 nx ny nz), k2 = kx^2+ky^2+kz^2
+            // in FP64, stored back in the spectrum precision; zero mode 0.
+            // One refined hardware reciprocal per element (within 1 ulp of
+            // 1/x) replaces the reference's two divisions per component.
+            const int kxi = NYQ ? N / 2 : bxi * C + c;
+            const int kyi = NYQ ? bxi * C + c : byi;
+            const double kxy = skk[kxi] + skk[kyi];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int kz = spec_k(r);
+                if (!NYQ && kxi == 0 && kyi == 0 && kz == 0) {
+                    v[r].x = (T)0; v[r].y = (T)0;
+                } else {
+                    const double s = -rcp_refined((kxy + skk[kz]) * ntot);
+                    v[r].x = (T)((double)v[r].x * s);
+                    v[r].y = (T)((double)v[r].y * s);
+                }
+            }
+            inverse();
+        }
+        if constexpr (MODE == 0) {
+#pragma unroll
+            for (int r = 0; r < 8; r++) st(g + (size_t)spec_k(r) * sa + c * sc, v[r]);
+        } else {
+#pragma unroll
+            for (int r = 0; r < 8; r++) st(g + (size_t)(t + r * TPC) * sa + c * sc, v[r]);
+        }
+        return;
+    }
     // Register-direct form (strided columns, N/8 threads per column, radix-8
     // last stage): thread t of column c loads and stores elements t + r N/8,
     // which are the inputs of its first-stage butterfly and the outputs of
@@ -274,7 +413,7 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
         const int c = tid % C, t = tid / C;
         V v[8];
 #pragma unroll
-        for (int r = 0; r < 8; r++) v[r] = gload(g + (size_t)(t + r * TPC) * sa + c * sc);
+        for (int r = 0; r < 8; r++) v[r] = ld(g + (size_t)(t + r * TPC) * sa + c * sc);
         stage_twiddles<V, N, NT>(stw, tw, tid);
         __shared__ double skk[MODE == 2 ? N : 1];   // per-axis k^2, staged with the data
         if constexpr (MODE == 2) {
@@ -318,7 +457,7 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
             fft_last8_from_lds<V, N, LD, true>(lds, c, t, v, stw);
         }
 #pragma unroll
-        for (int r = 0; r < 8; r++) gstore(g + (size_t)(t + r * TPC) * sa + c * sc, v[r]);
+        for (int r = 0; r < 8; r++) st(g + (size_t)(t + r * TPC) * sa + c * sc, v[r]);
         return;
     }
     {
@@ -328,7 +467,7 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
             const int e = tid + r * NT;
             int a, c;
             if (AXIS_CONTIG) { a = e % N; c = e / N; } else { c = e % C; a = e / C; }
-            buf[r] = gload(g + a * sa + c * sc);
+            buf[r] = ld(g + a * sa + c * sc);
         }
         stage_twiddles<V, N, NT>(stw, tw, tid);
 #pragma unroll
@@ -384,7 +523,7 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
         const int e = tid + r * NT;
         int a, cc;
         if (AXIS_CONTIG) { a = e % N; cc = e / N; } else { cc = e % C; a = e / C; }
-        gstore(g + a * sa + cc * sc, lds[a * LD + cc]);
+        st(g + a * sa + cc * sc, lds[a * LD + cc]);
     }
 }
 
@@ -436,6 +575,21 @@ struct PoissonPasses {
         else k_fft_x_fwd<T, N, X_IN_FIELD><<<grid, block, 0, st>>>(one, tab, nb);
         return MHIT_OK;
     }
+    // A column pass in the context's transform form (Tuning::fft_wave):
+    // the wavefront form where it fits, else the register-direct / LDS
+    // form.  Boxes of one batched launch share the form.
+    template <int C, bool AC, int MODE, bool NYQ>
+    static void launch_cols(const mhit_ctx* c, dim3 grid, const ColArgs<T>& one, const ColArgs<T>* tab,
+                            int nb, hipStream_t st) {
+        const dim3 block(C * Cfg::TPC);
+        if constexpr (wave_form_fits<T, N, C, AC>()) {
+            if (c->tune.fft_wave) {
+                k_fft_cols<T, N, C, AC, MODE, NYQ, true><<<grid, block, 0, st>>>(one, tab, nb);
+                return;
+            }
+        }
+        k_fft_cols<T, N, C, AC, MODE, NYQ, false><<<grid, block, 0, st>>>(one, tab, nb);
+    }
     template <int MODE>
     static int y_pass(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
         constexpr int CY = Cfg::CY, CN = Cfg::CN;
@@ -447,16 +601,14 @@ struct PoissonPasses {
                               (const V*)c->tw, c->kk, 0.0, c->S};
         }, one, tab);
         if (rc) return rc;
-        k_fft_cols<T, N, CY, false, MODE, false>
-            <<<dim3(H / CY, N, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
+        launch_cols<CY, false, MODE, false>(bx[0], dim3(H / CY, N, nb), one, tab, nb, st);
         // nyq [z][y]: columns z, axis y contiguous
         rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return ColArgs<T>{(V*)c->nyq, (size_t)CN * N, 0, 1, (size_t)N,
                               (const V*)c->tw, c->kk, 0.0, c->S};
         }, one, tab);
         if (rc) return rc;
-        k_fft_cols<T, N, CN, true, MODE, true>
-            <<<dim3(N / CN, 1, nb), dim3(CN * Cfg::TPC), 0, st>>>(one, tab, nb);
+        launch_cols<CN, true, MODE, true>(bx[0], dim3(N / CN, 1, nb), one, tab, nb, st);
         return MHIT_OK;
     }
     template <int MODE>
@@ -477,16 +629,14 @@ struct PoissonPasses {
                               (const V*)c->tw, c->kk, ntot, c->S};
         }, one, tab);
         if (rc) return rc;
-        k_fft_cols<T, N, CY, false, MODE, false>
-            <<<dim3(H / CY, N, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
+        launch_cols<CY, false, MODE, false>(bx[0], dim3(H / CY, N, nb), one, tab, nb, st);
         // nyq [z][y]: columns y, axis z (stride N)
         rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return ColArgs<T>{(V*)c->nyq, (size_t)CY, 0, (size_t)N, 1,
                               (const V*)c->tw, c->kk, ntot, c->S};
         }, one, tab);
         if (rc) return rc;
-        k_fft_cols<T, N, CY, false, MODE, true>
-            <<<dim3(N / CY, 1, nb), dim3(CY * Cfg::TPC), 0, st>>>(one, tab, nb);
+        launch_cols<CY, false, MODE, true>(bx[0], dim3(N / CY, 1, nb), one, tab, nb, st);
         return MHIT_OK;
     }
     static int x_inv(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
@@ -607,8 +757,10 @@ int poisson_project(mhit_ctx* c) {
 int poisson_project_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipStream_t st) {
     mhit_ctx* c = boxes[0];
     for (int b = 0; b < nb; b++)
-        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || !boxes[b]->march) {
-            set_error("batched projection: boxes differ in size or precision, or do not use the fused passes");
+        if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || !boxes[b]->march ||
+            boxes[b]->tune.fft_wave != c->tune.fft_wave) {
+            set_error("batched projection: boxes differ in size, precision or transform form, or do not "
+                      "use the fused passes");
             return MHIT_ERR_ARG;
         }
     return c->fp32 ? dispatch_batch<float>(boxes, nb, arena, st) : dispatch_batch<double>(boxes, nb, arena, st);

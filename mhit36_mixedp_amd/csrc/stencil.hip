@@ -325,11 +325,18 @@ void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
                 if ((W * H) % NT == 0 || tid + t * NT < W * H) pf[t] = pl[off[t]];
         };
         double pkm = live ? (double)p[idx3(i, j, wrap_m(z0, n), n)] : 0.0;
+        // u* of the own column one plane ahead: its loads have a whole
+        // plane of work to arrive, as the p plane's have
+        const size_t q0 = idx3(live ? i : 0, live ? j : 0, z0, n), plane = (size_t)n * n;
+        double usn = 0.0, vsn = 0.0, wsn = 0.0;
+        if (live) { usn = ld_once(&us[q0]); vsn = ld_once(&vs[q0]); wsn = ld_once(&ws[q0]); }
         fetch(z0);
         for (int k = z0; k < z1; k++) {
-            const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
-            double usq = 0.0, vsq = 0.0, wsq = 0.0;
-            if (live) { usq = us[q]; vsq = vs[q]; wsq = ws[q]; }   // in flight across the barrier
+            const size_t q = q0 + (size_t)(k - z0) * plane;
+            const double usq = usn, vsq = vsn, wsq = wsn;
+            if (live && k + 1 < z1) {
+                usn = ld_once(&us[q + plane]); vsn = ld_once(&vs[q + plane]); wsn = ld_once(&ws[q + plane]);
+            }
 #pragma unroll
             for (int t = 0; t < NP; t++) {
                 const int e = tid + t * NT;
@@ -343,7 +350,7 @@ void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
                 const double uu = usq - dtr * (pc - (double)sp[c - 1]) * g.dxi;
                 const double vv = vsq - dtr * (pc - (double)sp[c - W]) * g.dxi;
                 const double ww = wsq - dtr * (pc - pkm) * g.dxi;
-                u[q] = uu; v[q] = vv; w[q] = ww;
+                st_once(&u[q], uu); st_once(&v[q], vv); st_once(&w[q], ww);
                 su += uu; sv += vv; sw += ww;
                 mu_ = fmax(mu_, uu); mv_ = fmax(mv_, vv); mw_ = fmax(mw_, ww);
                 pkm = pc;
@@ -453,7 +460,10 @@ void k_finalize(FinArgs one, const FinArgs* __restrict__ tab) {
     MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     __shared__ double red[6][1024];
     const int t = threadIdx.x;
-    if (S->halted) return;
+    if (S->halted) {
+        if (t == 0) S->skipped += 1;   // a step whose passes all returned at once
+        return;
+    }
     double a[6] = {0.0, 0.0, 0.0, -INFINITY, -INFINITY, -INFINITY};
     for (int b = t; b < nblocks; b += 1024) {
         for (int f = 0; f < 3; f++) a[f] += partials[(size_t)b * 6 + f];

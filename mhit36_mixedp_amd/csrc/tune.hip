@@ -11,6 +11,10 @@
 // period (sort_every) is settable but not tuned: its pay-off depends on how
 // far tracers move, which a short trial does not see.
 //
+// The column transforms' form (fft_wave) is tuned as well: the wavefront
+// form and the register-direct form compute the same DFT in a different
+// operation order, so that choice changes results at the rounding level.
+//
 // mhit_autotune times candidate values one parameter at a time on the
 // context's own state, with the production path (captured graphs), and
 // restores the state after every trial: the run continues as if the tuner
@@ -150,13 +154,14 @@ void to_public(const Tuning& t, mhit_tuning* out, double ms) {
     out->sort_every = t.sort_every;
     out->ns_my = t.ns_my;
     out->ac_my = t.ac_my;
+    out->fft_wave = t.fft_wave;
     out->ms_per_step = ms;
 }
 
 bool valid(const mhit_tuning* t) {
     auto rows = [](int r) { return r == 8 || r == 16; };
     return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
-           rows(t->ns_my) && rows(t->ac_my);
+           rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1);
 }
 
 }  // namespace
@@ -177,6 +182,7 @@ int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
     c->tune.sort_every = t->sort_every;
     c->tune.ns_my = t->ns_my;
     c->tune.ac_my = t->ac_my;
+    c->tune.fft_wave = t->fft_wave;
     clear_graphs(c);
     return MHIT_OK;
 }
@@ -205,6 +211,7 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
         {&c->tune.ac_my, {16, 8}, c->P.phiflag != 0},
         {&c->tune.ac_mz, zlist({4, 8, 16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
+        {&c->tune.fft_wave, {1, 0}, true},
     };
     double best_ms = 0.0;
     rc = MHIT_OK;

@@ -115,3 +115,33 @@ def test_poisson_vs_oracle(n, prec, k2, oracle_mod):
     ref = o.poisson(rhs)
     tol = 1e-12 if prec == 0 else 1e-5
     assert relerr(p, ref) < tol, relerr(p, ref)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["direct", "wavefront"])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [64, 128, 256])
+def test_both_transform_forms_match_numpy(n, prec, form):
+    """Both forms of the column transforms (Tuning::fft_wave) against
+    numpy: forward (r2c: x, y, z passes), inverse (c2r) and the z pass's
+    forward + divide + inverse (poisson_solve against the spectral
+    solution p^ = -r^/|k|^2 computed with numpy)."""
+    print(f"[fft forms] n={n} prec={prec} form={form}", flush=True)
+    rng = np.random.default_rng(300 + n + form)
+    x = rng.standard_normal((n, n, n))
+    tol = 1e-12 if prec == 0 else 2e-6
+    with _solver(n, prec) as s:
+        s.set_tuning(fft_wave=form)
+        assert s.tuning()["fft_wave"] == form
+        X = s.fft_r2c(x)
+        assert relerr(X, np.fft.rfftn(x)) < tol
+        y = s.fft_c2r(np.fft.rfftn(x))
+        assert relerr(y, x * n ** 3) < tol
+        p = s.poisson_solve(x)
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kz, ky, kx = np.meshgrid(k, k, k[: n // 2 + 1], indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k2[0, 0, 0] = 1.0
+    ph = -np.fft.rfftn(x) / k2
+    ph[0, 0, 0] = 0.0
+    ref = np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2))
+    assert relerr(p, ref) < (1e-12 if prec == 0 else 1e-5)

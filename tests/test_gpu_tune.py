@@ -1,5 +1,6 @@
-"""The launch-shape autotuner leaves the state as it found it, and no
-launch shape changes a result bit."""
+"""The autotuner leaves the state as it found it, and no launch shape
+changes a result bit (the transform form, fft_wave, changes rounding
+only; tests/test_gpu_fft.py checks both forms)."""
 import numpy as np
 import pytest
 
@@ -23,9 +24,12 @@ def test_autotune_restores_state_and_results_are_shape_independent():
             s.step(2)
         t = a.autotune(steps=2)
         print(t, flush=True)
+        # the transform form changes rounding (and is a no-op at 48^3); the
+        # launch shapes change no bit
+        b.set_tuning(fft_wave=t["fft_wave"])
         assert t["ms_per_step"] > 0
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
-        assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16)
+        assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16) and t["fft_wave"] in (0, 1)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
         # every field, the last projection's pressure (4) included
         for f in range(1, 10):
@@ -72,10 +76,11 @@ def test_mhit_tune_overrides_and_ignores_bad_values(monkeypatch):
     print("[tuning] MHIT_TUNE parsing", flush=True)
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         base = s.tuning()
-    monkeypatch.setenv("MHIT_TUNE", "ns_mz=16,ac_mz=8,ns_my=16,ac_my=16,xcd=1,sort=3")
+    monkeypatch.setenv("MHIT_TUNE", "ns_mz=16,ac_mz=8,ns_my=16,ac_my=16,xcd=1,sort=3,fft=0")
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         t = s.tuning()
-    assert (t["ns_mz"], t["ac_mz"], t["ns_my"], t["ac_my"], t["xcd"], t["sort_every"]) == (16, 8, 16, 16, 1, 3)
+    assert (t["ns_mz"], t["ac_mz"], t["ns_my"], t["ac_my"], t["xcd"], t["sort_every"], t["fft_wave"]) == \
+        (16, 8, 16, 16, 1, 3, 0)
     monkeypatch.setenv("MHIT_TUNE", "ns_mz=0,ac_mz=-4,ns_my=12,ac_my=4,bogus=7,sort=-1")
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         t = s.tuning()
