@@ -325,18 +325,11 @@ void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
                 if ((W * H) % NT == 0 || tid + t * NT < W * H) pf[t] = pl[off[t]];
         };
         double pkm = live ? (double)p[idx3(i, j, wrap_m(z0, n), n)] : 0.0;
-        // u* of the own column one plane ahead: its loads have a whole
-        // plane of work to arrive, as the p plane's have
-        const size_t q0 = idx3(live ? i : 0, live ? j : 0, z0, n), plane = (size_t)n * n;
-        double usn = 0.0, vsn = 0.0, wsn = 0.0;
-        if (live) { usn = ld_once(&us[q0]); vsn = ld_once(&vs[q0]); wsn = ld_once(&ws[q0]); }
         fetch(z0);
         for (int k = z0; k < z1; k++) {
-            const size_t q = q0 + (size_t)(k - z0) * plane;
-            const double usq = usn, vsq = vsn, wsq = wsn;
-            if (live && k + 1 < z1) {
-                usn = ld_once(&us[q + plane]); vsn = ld_once(&vs[q + plane]); wsn = ld_once(&ws[q + plane]);
-            }
+            const size_t q = idx3(live ? i : 0, live ? j : 0, k, n);
+            double usq = 0.0, vsq = 0.0, wsq = 0.0;
+            if (live) { usq = ld_once(&us[q]); vsq = ld_once(&vs[q]); wsq = ld_once(&ws[q]); }   // in flight across the barrier
 #pragma unroll
             for (int t = 0; t < NP; t++) {
                 const int e = tid + t * NT;
