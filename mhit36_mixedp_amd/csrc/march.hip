@@ -394,6 +394,7 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
     const double alpha = S->alpha, beta = S->beta;
     const double gamma = 1.0 * S->gumax;               // main.f90:536
     const double dxi = g.dxi, ddxi = g.ddxi, eps = g.eps, enm = g.enm, dt = g.dt;
+    const double hdx = 0.5 * dxi, ced = eps * ddxi;
 
     // phi of one plane in flight: global -> registers
     constexpr int NPL = (H2 * W2 + NT - 1) / NT;
@@ -445,10 +446,10 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         // hardware reciprocal (within 1 ulp)
         const double normag = rcp_refined(sqrt(nx * nx + ny * ny + nz * nz) + enm);
         const double val = fmin(PH(k, a, b), 1.0);
-        const double G = (val + enm) * (1.0 - val + enm) * g_scale;
-        fx = G * (nx * normag);
-        fy = G * (ny * normag);
-        fz = G * (nz * normag);
+        const double Gn = (val + enm) * (1.0 - val + enm) * g_scale * normag;
+        fx = Gn * nx;
+        fy = Gn * ny;
+        fz = Gn * nz;
     };
     // sharpening fluxes of plane k into buffer bf (tile + 1-point halo in x / y);
     // returns Fz at the thread's column
@@ -518,19 +519,20 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
             const double pxp = PH(k, tx + 1, ty), pxm = PH(k, tx - 1, ty);
             const double pyp = PH(k, tx, ty + 1), pym = PH(k, tx, ty - 1);
             const double pzp = PH(k + 1, tx, ty), pzm = PH(k - 1, tx, ty);
-            // P2 conv + diff, main.f90:551-557
-            double r = -(up * 0.5 * (pxp + pc) - uc * 0.5 * (pc + pxm)) * dxi
-                       - (vp * 0.5 * (pyp + pc) - vc * 0.5 * (pc + pym)) * dxi
-                       - (wp * 0.5 * (pzp + pc) - wk * 0.5 * (pc + pzm)) * dxi
-                       + gamma * (eps * (pxp - 2.0 * pc + pxm) * ddxi +
-                                  eps * (pyp - 2.0 * pc + pym) * ddxi +
-                                  eps * (pzp - 2.0 * pc + pzm) * ddxi);
-            // P4 sharpening, main.f90:609-614
+            // P2 conv + diff (main.f90:551-557) and P4 sharpening
+            // (609-614) with the common factors taken out of the sums:
+            // 0.5/dx of the face fluxes and of the sharpening divergence,
+            // eps/dx^2 of the diffusion.  Each face flux is still formed
+            // from the same operands by the two cells that share it;
+            // the sum differs from the term-by-term form by a few ulp.
             const double* Fx = sFx[bf];
             const double* Fy = sFy[bf];
-            r = r - gamma * ((Fx[ty * (MX + 2) + tx + 2] - Fx[ty * (MX + 2) + tx]) * 0.5 * dxi +
-                             (Fy[(ty + 2) * MX + tx] - Fy[ty * MX + tx]) * 0.5 * dxi +
-                             (fz_p1 - fz_m1) * 0.5 * dxi);
+            const double conv = ((up * (pxp + pc) - uc * (pc + pxm)) + (vp * (pyp + pc) - vc * (pc + pym))) +
+                                (wp * (pzp + pc) - wk * (pc + pzm));
+            const double lap = ((pxp + pxm) + (pyp + pym)) + (pzp + pzm) - 6.0 * pc;
+            const double shp = ((Fx[ty * (MX + 2) + tx + 2] - Fx[ty * (MX + 2) + tx]) +
+                                (Fy[(ty + 2) * MX + tx] - Fy[ty * MX + tx])) + (fz_p1 - fz_m1);
+            const double r = gamma * (ced * lap - hdx * shp) - hdx * conv;
             // P5 AB2, main.f90:625-626
             st_once(&phi_out[q], pc + dt * (alpha * r - beta * ro));
             st_once(&rphi_o[q], r);
