@@ -192,6 +192,13 @@ __device__ __forceinline__ double rcp_refined(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+// 1/sqrt(x) for a normal x > 0: the hardware estimate (v_rsq_f64) refined
+// by one Newton step, r' = r + r/2 (1 - x r^2), in four operations.
+__device__ __forceinline__ double rsq_refined(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    return fma(0.5 * r, fma(-x * r, r, 1.0), r);
+}
+
 // ln(a/b) for a, b > 0 without rounding the quotient first, in ~40 VALU
 // operations (one reciprocal) instead of a division plus a library log.
 // Method: a = ma 2^ea, b = mb 2^eb (frexp); move a factor 2 between ma and

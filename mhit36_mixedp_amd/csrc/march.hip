@@ -442,9 +442,11 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         const double nx = (PS(k, a + 1, b) - PS(k, a - 1, b));
         const double ny = (PS(k, a, b + 1) - PS(k, a, b - 1));
         const double nz = (PS(k + 1, a, b) - PS(k - 1, a, b));
-        // 1/(|grad psi| + 1e-16), main.f90:559-561, with the refined
-        // hardware reciprocal (within 1 ulp)
-        const double normag = rcp_refined(sqrt(nx * nx + ny * ny + nz * nz) + enm);
+        // 1/(|grad psi| + 1e-16), main.f90:559-561: |grad psi| as s/sqrt(s)
+        // with the refined hardware reciprocal square root, then the
+        // refined hardware reciprocal (each within 1 ulp)
+        const double s2 = fma(nx, nx, fma(ny, ny, nz * nz));
+        const double normag = rcp_refined((s2 > 0.0 ? s2 * rsq_refined(s2) : 0.0) + enm);
         const double val = fmin(PH(k, a, b), 1.0);
         const double Gn = (val + enm) * (1.0 - val + enm) * g_scale * normag;
         fx = Gn * nx;
