@@ -325,11 +325,19 @@ int step_async(mhit_ctx* c, int nsteps) {
     return MHIT_OK;
 }
 
+int finish_synced(mhit_ctx* c);
+
 // Wait for the queued steps and read the scalars back; MHIT_ERR_CFL if the
 // CFL check stopped the run (main.f90:1006).
 int finish_steps(mhit_ctx* c) {
     int rc;
     if ((rc = sync_scalars(c))) return rc;
+    return finish_synced(c);
+}
+
+// finish_steps once S_host holds the device scalars
+int finish_synced(mhit_ctx* c) {
+    int rc;
     if (c->S_host.skipped) {
         // steps queued behind a CFL stop wrote no AB2 history but the host
         // still swapped the history buffers after each: an odd count leaves

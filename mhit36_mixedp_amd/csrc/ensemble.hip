@@ -24,6 +24,7 @@
 namespace mhit {
 int step_async(mhit_ctx* c, int nsteps);     // api.hip: enqueue without waiting
 int finish_steps(mhit_ctx* c);               // api.hip: wait, read the scalars back
+int finish_synced(mhit_ctx* c);              // api.hip: the same once S_host is read
 int step_eager_part(mhit_ctx* c);            // api.hip: tracer re-ordering if due
 int step_kernels(mhit_ctx* c);               // api.hip: the kernels of one step
 void clear_graphs(mhit_ctx* c);              // api.hip: drop a box's captured steps
@@ -54,7 +55,38 @@ struct mhit_ensemble {
     bool batched = true;                    // one launch per pass and grid size
     std::vector<hipStream_t> size_stream;   // graph branch of each grid size
     std::vector<hipEvent_t> size_join;
+    // the boxes' scalars read back in one copy: a device table of their
+    // DevScalars pointers, a device gather buffer and its pinned host copy
+    const DevScalars** s_src = nullptr;
+    DevScalars* s_dev = nullptr;
+    DevScalars* s_host = nullptr;
 };
+
+__global__ void k_gather_scalars(const DevScalars* const* __restrict__ src, DevScalars* __restrict__ dst,
+                                 int nb) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nb) dst[b] = *src[b];
+}
+
+// After the replays on the ensemble stream: every box's scalars to the host
+// with one gather kernel and one copy (one round trip instead of one per
+// box), then each box's own end-of-call bookkeeping.
+static int finish_ensemble(mhit_ensemble* e, int* status) {
+    const int nb = (int)e->boxes.size();
+    k_gather_scalars<<<div_up(nb, 64), 64, 0, e->stream>>>(e->s_src, e->s_dev, nb);
+    MHIT_HIP_TRY(hipGetLastError());
+    MHIT_HIP_TRY(hipMemcpyAsync(e->s_host, e->s_dev, nb * sizeof(DevScalars), hipMemcpyDeviceToHost, e->stream));
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+    *status = MHIT_OK;
+    for (int b = 0; b < nb; b++) {
+        mhit_ctx* c = e->boxes[b];
+        c->S_host = e->s_host[b];
+        const int rc = finish_synced(c);
+        if (rc == MHIT_ERR_CFL) *status = MHIT_ERR_CFL;
+        else if (rc) return rc;
+    }
+    return MHIT_OK;
+}
 
 // The caller has synchronised the ensemble stream (no replay in flight).
 static void clear_ensemble_graphs(mhit_ensemble* e) {
@@ -324,6 +356,12 @@ int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble*
               hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess;
     for (auto& ev : e->join)
         ok = ok && hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
+    std::vector<const DevScalars*> src(nbox);
+    for (int32_t b = 0; b < nbox; b++) src[b] = e->boxes[b]->S;
+    ok = ok && hipMalloc((void**)&e->s_src, nbox * sizeof(DevScalars*)) == hipSuccess &&
+         hipMalloc((void**)&e->s_dev, nbox * sizeof(DevScalars)) == hipSuccess &&
+         hipHostMalloc((void**)&e->s_host, nbox * sizeof(DevScalars), hipHostMallocDefault) == hipSuccess &&
+         hipMemcpy(e->s_src, src.data(), nbox * sizeof(DevScalars*), hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         set_error("ensemble stream/event creation failed");
         mhit_ensemble_destroy(e);
@@ -355,7 +393,10 @@ int mhit_ensemble_step(mhit_ensemble* e, int nsteps) {
             Range r("Iteration");
             if ((rc = ensemble_graph_step(e))) return rc;
         }
-        MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+        int status = MHIT_OK;
+        if ((rc = finish_ensemble(e, &status))) return rc;
+        if (status == MHIT_ERR_CFL) set_error("at least one box stopped on cou > 7");
+        return status;
     } else {
         // one stream per box, interleaved step by step so that no stream
         // runs far ahead (a box already stopped on the CFL check is skipped)
@@ -401,6 +442,9 @@ int mhit_ensemble_destroy(mhit_ensemble* e) {
         if (ev) (void)hipEventDestroy(ev);
     if (e->fork) (void)hipEventDestroy(e->fork);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    if (e->s_src) (void)hipFree(e->s_src);
+    if (e->s_dev) (void)hipFree(e->s_dev);
+    if (e->s_host) (void)hipHostFree(e->s_host);
     delete e;
     return MHIT_OK;
 }
