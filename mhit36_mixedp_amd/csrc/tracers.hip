@@ -8,7 +8,6 @@
 // so it has no counterpart here; tracers keep their order and their tag.
 #include "internal.hpp"
 
-#include <hipcub/hipcub.hpp>
 #include <utility>
 
 namespace mhit {
@@ -132,6 +131,157 @@ void k_tracer_permute(const int* __restrict__ perm, Cols7 src, Cols7 dst, long l
     for (int t = 0; t < 7; t++) dst.c[t][p] = src.c[t][s];
 }
 
+// Stable LSD radix sort of (key, index) pairs, 8 bits per pass.  A
+// workgroup of 256 threads owns a tile of 4096 consecutive keys; wave w
+// holds the quarter [1024 w, 1024 w + 1024) in 16 rows of 64 lanes, so the
+// tile order is (wave, row, lane).  Per pass:
+//   k_sort_hist     digit counts of each tile -> counts[digit][tile]
+//   k_sort_scan     one workgroup per digit: exclusive prefix of its row
+//                   over the tiles, in place, and the digit's total
+//   k_sort_scatter  each key goes to (keys of smaller digits) + (its
+//                   digit's keys in earlier tiles) + its rank among the
+//                   tile's earlier keys of the same digit (lanes of one
+//                   digit found by ballots over its 8 bits; running counts
+//                   per wave in LDS, then summed over the earlier waves)
+// Equal digits keep their order, so the passes compose to a stable sort.
+constexpr int kSortThreads = 256, kSortRows = 16, kSortTile = kSortThreads * kSortRows;
+constexpr int kSortDigits = 256, kSortWaves = kSortThreads / 64;
+
+__global__ __launch_bounds__(kSortThreads)
+void k_sort_hist(const unsigned* __restrict__ keys, unsigned* __restrict__ counts, long long np, int shift,
+                 int ntile) {
+    __shared__ unsigned h[kSortDigits];
+    const int tid = threadIdx.x;
+    h[tid] = 0u;
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile;
+#pragma unroll
+    for (int r = 0; r < kSortRows; r++) {
+        const long long p = base + r * kSortThreads + tid;
+        if (p < np) atomicAdd(&h[(keys[p] >> shift) & 0xffu], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)tid * ntile + blockIdx.x] = h[tid];
+}
+
+// Exclusive scan of a digit's row over the tiles (blockIdx.x = digit), in
+// chunks of 4 entries per thread; the row's total goes to totals[digit].
+__global__ __launch_bounds__(kSortThreads)
+void k_sort_scan(unsigned* __restrict__ counts, unsigned* __restrict__ totals, int ntile) {
+    __shared__ unsigned wsum[kSortWaves];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    unsigned* row = counts + (size_t)blockIdx.x * ntile;
+    unsigned carry = 0u;
+    for (int c0 = 0; c0 < ntile; c0 += 4 * kSortThreads) {
+        unsigned x[4];
+        unsigned s = 0u;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = c0 + 4 * tid + q;
+            x[q] = e < ntile ? row[e] : 0u;
+            s += x[q];
+        }
+        unsigned incl = s;   // inclusive scan of the thread sums over the wave
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned t = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += t;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        unsigned before = carry, all = 0u;
+#pragma unroll
+        for (int v = 0; v < kSortWaves; v++) {
+            if (v < w) before += wsum[v];
+            all += wsum[v];
+        }
+        before += incl - s;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int e = c0 + 4 * tid + q;
+            if (e < ntile) row[e] = before;
+            before += x[q];
+        }
+        carry += all;
+        __syncthreads();   // wsum is reused by the next chunk
+    }
+    if (tid == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kSortThreads)
+void k_sort_scatter(const unsigned* __restrict__ kin, const int* __restrict__ vin, unsigned* __restrict__ kout,
+                    int* __restrict__ vout, const unsigned* __restrict__ counts, const unsigned* __restrict__ totals,
+                    long long np, int shift, int ntile) {
+    // running digit counts of each wave, then the wave's offsets in the tile
+    __shared__ unsigned wcnt[kSortWaves][kSortDigits];
+    __shared__ unsigned toff[kSortDigits];   // output offset of the tile's run of each digit
+    __shared__ unsigned wsum[kSortWaves];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int v = 0; v < kSortWaves; v++) wcnt[v][tid] = 0u;
+    {   // keys of smaller digits: exclusive scan of the totals (thread = digit)
+        const unsigned t = totals[tid];
+        unsigned incl = t;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned u = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += u;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        unsigned before = incl - t;
+        for (int v = 0; v < w; v++) before += wsum[v];
+        toff[tid] = before + counts[(size_t)tid * ntile + blockIdx.x];
+    }
+    __syncthreads();
+    const long long base = (long long)blockIdx.x * kSortTile + (long long)w * (kSortTile / kSortWaves);
+    unsigned key[kSortRows], rank[kSortRows];
+    int val[kSortRows];
+#pragma unroll
+    for (int r = 0; r < kSortRows; r++) {
+        const long long p = base + r * 64 + lane;
+        key[r] = p < np ? kin[p] : 0u;
+        val[r] = p < np ? vin[p] : 0;
+    }
+    const unsigned long long below = (1ull << lane) - 1ull;
+    // the leader's update must reach the next row's reads in program order
+    volatile unsigned* wc = wcnt[w];
+#pragma unroll
+    for (int r = 0; r < kSortRows; r++) {
+        const bool ok = base + r * 64 + lane < np;
+        const unsigned d = (key[r] >> shift) & 0xffu;
+        unsigned long long peers = __ballot(ok);
+#pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const bool set = (d >> bit) & 1u;
+            const unsigned long long m = __ballot(set);
+            peers &= set ? m : ~m;
+        }
+        const unsigned before = wc[d];
+        rank[r] = before + (unsigned)__popcll(peers & below);
+        if (ok && (peers & below) == 0ull) wc[d] = before + (unsigned)__popcll(peers);
+    }
+    __syncthreads();
+    {   // thread tid = digit: exclusive sum over the waves
+        unsigned run = 0u;
+#pragma unroll
+        for (int v = 0; v < kSortWaves; v++) {
+            const unsigned t = wcnt[v][tid];
+            wcnt[v][tid] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSortRows; r++) {
+        if (base + r * 64 + lane >= np) continue;
+        const unsigned d = (key[r] >> shift) & 0xffu;
+        const unsigned pos = toff[d] + wcnt[w][d] + rank[r];
+        kout[pos] = key[r];
+        vout[pos] = val[r];
+    }
+}
+
 void free_sort_buffers(mhit_ctx* c) {
     void* ptrs[] = {c->skey[0], c->skey[1], c->sidx[0], c->sidx[1], c->sort_tmp,
                     c->alt[0], c->alt[1], c->alt[2], c->alt[3], c->alt[4], c->alt[5], c->alt[6]};
@@ -148,7 +298,8 @@ void free_sort_buffers(mhit_ctx* c) {
 int sort_tracers(mhit_ctx* c) {
     const long long np = c->np;
     if (np <= 1) return MHIT_OK;
-    const int bits = 3 * 10;
+    if (np > (long long)INT32_MAX) { set_error("tracer sort: more than 2^31 - 1 tracers"); return MHIT_ERR_ARG; }
+    const int ntile = div_up(np, kSortTile);
     if (np > c->sort_cap) {
         free_sort_buffers(c);
         for (int t = 0; t < 2; t++) {
@@ -156,23 +307,31 @@ int sort_tracers(mhit_ctx* c) {
             MHIT_HIP_TRY(hipMalloc((void**)&c->sidx[t], np * sizeof(int)));
         }
         for (double*& a : c->alt) MHIT_HIP_TRY(hipMalloc((void**)&a, np * sizeof(double)));
-        size_t tmp = 0;
-        MHIT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, c->skey[0], c->skey[1], c->sidx[0],
-                                                        c->sidx[1], (int)np, 0, bits, c->stream));
-        MHIT_HIP_TRY(hipMalloc(&c->sort_tmp, tmp));
-        c->sort_tmp_bytes = tmp;
+        c->sort_tmp_bytes = (size_t)kSortDigits * (ntile + 1) * sizeof(unsigned);
+        MHIT_HIP_TRY(hipMalloc(&c->sort_tmp, c->sort_tmp_bytes));
         c->sort_cap = np;
     }
+    unsigned* counts = (unsigned*)c->sort_tmp;               // [digit][tile]
+    unsigned* totals = counts + (size_t)kSortDigits * ntile;   // [digit]
     const int blocks = div_up(np, 256);
     k_tracer_keys<<<blocks, 256, 0, c->stream>>>(c->px, c->py, c->pz, c->skey[0], c->sidx[0], np, c->g);
     MHIT_HIP_TRY(hipGetLastError());
-    size_t tmp = c->sort_tmp_bytes;
-    MHIT_HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->sort_tmp, tmp, c->skey[0], c->skey[1], c->sidx[0],
-                                                    c->sidx[1], (int)np, 0, bits, c->stream));
+    // key bits in use: 3 per level of the cell index
+    int axis_bits = 0;
+    while ((1 << axis_bits) < c->n) axis_bits++;
+    int cur = 0;
+    for (int shift = 0; shift < 3 * axis_bits; shift += 8) {
+        k_sort_hist<<<ntile, kSortThreads, 0, c->stream>>>(c->skey[cur], counts, np, shift, ntile);
+        k_sort_scan<<<kSortDigits, kSortThreads, 0, c->stream>>>(counts, totals, ntile);
+        k_sort_scatter<<<ntile, kSortThreads, 0, c->stream>>>(c->skey[cur], c->sidx[cur], c->skey[cur ^ 1],
+                                                               c->sidx[cur ^ 1], counts, totals, np, shift, ntile);
+        MHIT_HIP_TRY(hipGetLastError());
+        cur ^= 1;
+    }
     double** cols[7] = {&c->ptag, &c->px, &c->py, &c->pz, &c->pu1, &c->pv1, &c->pw1};
     Cols7 src, dst;
     for (int t = 0; t < 7; t++) { src.c[t] = *cols[t]; dst.c[t] = c->alt[t]; }
-    k_tracer_permute<<<blocks, 256, 0, c->stream>>>(c->sidx[1], src, dst, np);
+    k_tracer_permute<<<blocks, 256, 0, c->stream>>>(c->sidx[cur], src, dst, np);
     MHIT_HIP_TRY(hipGetLastError());
     for (int t = 0; t < 7; t++) std::swap(*cols[t], c->alt[t]);
     prof_mark(c, ST_TRACER_SORT);
