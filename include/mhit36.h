@@ -188,7 +188,9 @@ int mhit_set_ab2_state(mhit_ctx* ctx, double alpha, double beta, double gumax);
 
 /* Launch shapes (no counterpart in the reference, whose start-up tunes
  * cuDecomp's process grid and backends instead, multi/main.f90:80-136).
- * None of these changes a result bit.                                     */
+ * None of these changes a result bit, except the FFT transform form
+ * (fft_wave, and widths that move a pass to or from eight columns), which
+ * changes rounding.                                                       */
 typedef struct mhit_tuning {
     int32_t ns_mz;        /* z planes per workgroup, fused NS pass          */
     int32_t ac_mz;        /* z planes per workgroup, fused ACDI pass        */
@@ -202,8 +204,21 @@ typedef struct mhit_tuning {
                              (n = 64, 128, 256, 512); 0: the register-direct
                              form with LDS exchanges.  The two forms compute
                              the same DFT in a different operation order, so
-                             they differ in rounding only; the other fields
-                             change no result bit.                          */
+                             they differ in rounding only.                  */
+    int32_t fft_ycols;    /* columns per workgroup of the y passes          */
+    int32_t fft_zcols;    /* columns per workgroup of the z pass            */
+    int32_t fft_xrows;    /* row pairs per workgroup of the x passes: the FFT
+                             launch widths.  0 in mhit_set_tuning keeps the
+                             default for the size and precision;
+                             mhit_get_tuning reports the width in use.  Grids
+                             of 64 and more have compiled alternatives: y 16
+                             or 8 columns (FP32 Poisson), 8 or 4 (FP64); z 8
+                             or 16 (FP32), 8 or 4 (FP64); x 8, 4 or 16 row
+                             pairs (FP32), 4, 2 or 8 (FP64).  Eight columns
+                             take the wavefront form when fft_wave is 1, so a
+                             width that moves a pass to or from 8 columns
+                             changes rounding; the other fields change no
+                             result bit.                                    */
     double ms_per_step;   /* mhit_autotune: measured with these settings    */
 } mhit_tuning;
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);

@@ -92,8 +92,10 @@ class Diag(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    """mhit_tuning: launch shapes (no result bit changes) and the column
-    transforms' form (fft_wave: rounding-level differences only)."""
+    """mhit_tuning: launch shapes (no result bit changes), the column
+    transforms' form (fft_wave: rounding-level differences only) and the FFT
+    launch widths (0 = the default; a width to or from eight columns moves
+    a pass to or from the wavefront form)."""
 
     _fields_ = [
         ("ns_mz", ctypes.c_int32),
@@ -103,6 +105,9 @@ class Tuning(ctypes.Structure):
         ("ns_my", ctypes.c_int32),
         ("ac_my", ctypes.c_int32),
         ("fft_wave", ctypes.c_int32),
+        ("fft_ycols", ctypes.c_int32),
+        ("fft_zcols", ctypes.c_int32),
+        ("fft_xrows", ctypes.c_int32),
         ("ms_per_step", ctypes.c_double),
     ]
 

@@ -92,6 +92,10 @@ static void parse_tuning(const char* s, Tuning& t) {
             else if (key == "xcd") t.xcd = val != 0;
             else if (key == "sort" && val >= 0) t.sort_every = val;
             else if (key == "fft") t.fft_wave = val != 0;
+            // widths that are not compiled for the size fall back to the default
+            else if (key == "ycols" && val > 0) t.fft_ycols = val;
+            else if (key == "zcols" && val > 0) t.fft_zcols = val;
+            else if (key == "xrows" && val > 0) t.fft_xrows = val;
         }
         pos = end + 1;
     }

@@ -43,8 +43,8 @@ struct DevScalars {
 // passes, XCD-aware tile order), which do not change results, and the form
 // of the column transforms, which changes them only by rounding (the same
 // DFT with a different operation order).  Defaults are set by grid size at
-// create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,ac_my=8,xcd=1,sort=10,fft=1"
-// overrides them.
+// create; MHIT_TUNE="ns_mz=64,ac_mz=64,ns_my=16,ac_my=8,xcd=1,sort=10,fft=1,
+// ycols=16,zcols=8,xrows=8" overrides them.
 struct Tuning {
     int ns_mz = 64;       // planes per workgroup, NS pass (set by grid size at create)
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
@@ -55,6 +55,11 @@ struct Tuning {
     int fft_wave = 1;     // column transforms of eight columns in the wavefront
                           // form (lane/register transposes by permlane and DPP)
                           // where n = 64, 128, 256, 512; 0: register-direct form
+    // FFT launch widths (0: the default for the size and precision; n >= 64
+    // has compiled alternatives, fft_widths in poisson.hip)
+    int fft_ycols = 0;    // columns per workgroup, y passes
+    int fft_zcols = 0;    // columns per workgroup, z pass
+    int fft_xrows = 0;    // row pairs per workgroup, x passes
 };
 
 // Per-run constants passed by value to kernels.

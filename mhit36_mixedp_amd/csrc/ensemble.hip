@@ -176,7 +176,9 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
     };
     for (auto& gp : group_by(boxes, ns_key))
         if ((rc = launch_ns_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
-    auto prec_key = [](mhit_ctx* c) { return std::make_pair(c->fp32, c->tune.fft_wave); };
+    auto prec_key = [](mhit_ctx* c) {
+        return std::make_tuple(c->fp32, c->tune.fft_wave, fft_width(c, 0), fft_width(c, 1), fft_width(c, 2));
+    };
     const auto by_prec = group_by(boxes, prec_key);
     for (auto& gp : by_prec)
         if ((rc = poisson_project_batch(gp.data(), (int)gp.size(), ar, st))) return rc;

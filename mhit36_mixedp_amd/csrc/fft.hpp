@@ -125,8 +125,19 @@ template <int R, bool INV, typename V> __device__ __forceinline__ void dft(V* a)
     else dft8<INV>(a);
 }
 
+// LDS index of element a of column c: rows of LD values.  With SW != 0
+// the column is XOR-swizzled by the row (c ^ ((a >> 1) & SW), SW = C - 1,
+// LD = C): the values a 16-lane group reads together (8-byte values; 8 lanes
+// for 16-byte values) then sit in distinct banks both when the lanes walk
+// along a row and when they walk down a column (the x passes do both).
+template <int LD, int SW>
+__device__ __forceinline__ int lidx(int a, int c) {
+    if constexpr (SW != 0) return a * LD + (c ^ ((a >> 1) & SW));
+    else return a * LD + c;
+}
+
 // One Stockham stage on C columns in LDS.
-template <typename V, int N, int R, int NS, int C, int TPC, int LD, bool INV>
+template <typename V, int N, int R, int NS, int C, int TPC, int LD, bool INV, int SW = 0>
 __device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restrict__ tw) {
     constexpr int NBF = N / R;
     constexpr int NB = (NBF + TPC - 1) / TPC;
@@ -136,7 +147,7 @@ __device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restr
         const int b = t + q * TPC;
         if (NBF % TPC == 0 || b < NBF) {
 #pragma unroll
-            for (int r = 0; r < R; r++) v[q][r] = lds[(b + r * NBF) * LD + c];
+            for (int r = 0; r < R; r++) v[q][r] = lds[lidx<LD, SW>(b + r * NBF, c)];
         }
     }
     __syncthreads();
@@ -156,19 +167,19 @@ __device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restr
             dft<R, INV>(v[q]);
             const int j = (b / NS) * NS * R + k;
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[(j + r * NS) * LD + c] = v[q][r];
+            for (int r = 0; r < R; r++) lds[lidx<LD, SW>(j + r * NS, c)] = v[q][r];
         }
     }
     __syncthreads();
 }
 
 // Stages from NS (points already combined) up to NS_END, each in LDS.
-template <typename V, int N, int NS, int C, int TPC, int LD, bool INV, int NS_END = N>
+template <typename V, int N, int NS, int C, int TPC, int LD, bool INV, int NS_END = N, int SW = 0>
 __device__ __forceinline__ void fft_stages(V* lds, int c, int t, const V* __restrict__ tw) {
     if constexpr (NS < NS_END) {
         constexpr int R = NextRadix<N / NS>::value;
-        fft_stage<V, N, R, NS, C, TPC, LD, INV>(lds, c, t, tw);
-        fft_stages<V, N, NS * R, C, TPC, LD, INV, NS_END>(lds, c, t, tw);
+        fft_stage<V, N, R, NS, C, TPC, LD, INV, SW>(lds, c, t, tw);
+        fft_stages<V, N, NS * R, C, TPC, LD, INV, NS_END, SW>(lds, c, t, tw);
     }
 }
 
@@ -275,8 +286,6 @@ template <typename T, int N> struct FftCfg {
     static constexpr bool F32 = sizeof(T) == 4;
     // columns per workgroup for the strided (y, z) passes: 128-byte rows
     static constexpr int CY = pow2_divisor_cap(H, F32 ? 16 : 8);
-    // columns per workgroup for the contiguous Nyquist-plane y pass
-    static constexpr int CN = pow2_divisor_cap(N, F32 ? 8 : 4);
     // row pairs per workgroup for the x passes
     static constexpr int PX = pow2_divisor_cap(H, F32 ? 8 : 4);
     // threads per column

@@ -209,6 +209,13 @@ void dump_release(mhit_ctx* c);
 
 // Supported grid sizes (FFT lengths).
 bool fft_size_supported(int n);
+// Launch widths compiled for an FFT pass (0: y passes, columns per
+// workgroup; 1: z pass, columns; 2: x passes, row pairs) of size n and
+// precision: out[0] is the default, the count is returned (at most 3).
+int fft_widths(int n, bool fp32, int pass, int out[3]);
+// The width a context's pass uses: its Tuning value if compiled, else the
+// default.
+int fft_width(const mhit_ctx* c, int pass);
 
 // poisson.hip
 enum class XSrc { Divergence, Real };

@@ -15,9 +15,10 @@
 // Passes per projection:
 //   X1 (x_fwd):   divergence of u* (main.f90:833-848) fused with the R2C
 //                 along x, two real rows per complex transform
-//   Y1, Y1n:      forward C2C along y on spec and on nyq
-//   Z:            forward C2C along z, divide (main.f90:883-896), inverse
-//   Y2, Y2n:      inverse C2C along y
+//   Y1:           forward C2C along y on spec and on nyq (one launch)
+//   Z:            forward C2C along z, divide (main.f90:883-896), inverse,
+//                 on spec and on nyq (one launch)
+//   Y2:           inverse C2C along y (one launch)
 //   X2 (x_inv):   C2R along x, writes pres
 #include "fft.hpp"
 #include "internal.hpp"
@@ -46,12 +47,12 @@ struct XFwdArgs {
     const cplx_t<T>* tw;
     const DevScalars* S;
 };
-template <typename T, int N, int IN>
-__global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
+template <typename T, int N, int IN, int P>
+__global__ __launch_bounds__((P * FftCfg<T, N>::TPC))
 void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
     using Cfg = FftCfg<T, N>;
-    constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
+    constexpr int TPC = Cfg::TPC, H = N / 2, LD = P, SW = P - 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
     __shared__ V stw[N];
@@ -85,7 +86,7 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
         if (IN == X_IN_READY) {
             val.x = ld_once(&din[idx3(a, y0 + 2 * c, z, N)]);
             val.y = ld_once(&din[idx3(a, y0 + 2 * c + 1, z, N)]);
-            lds[a * LD + c] = val;
+            lds[lidx<LD, SW>(a, c)] = val;
             continue;
         }
         double d[2];
@@ -107,18 +108,18 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
         }
         val.x = (T)d[0];   // rhspp = real(rhsp,4) in the FP32 form (main.f90:844)
         val.y = (T)d[1];
-        lds[a * LD + c] = val;
+        lds[lidx<LD, SW>(a, c)] = val;
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, false>(lds, tid % P, tid / P, stw);
+    fft_stages<V, N, 1, P, TPC, LD, false, N, SW>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (H * 2 * P + NT - 1) / NT; r++) {
         const int e = tid + r * NT;
         if ((H * 2 * P) % NT != 0 && e >= H * 2 * P) break;
         const int k = e % H, rr = e / H, cc = rr >> 1;
-        const V zk = lds[k * LD + cc];
-        const V zm = lds[((N - k) % N) * LD + cc];
+        const V zk = lds[lidx<LD, SW>(k, cc)];
+        const V zm = lds[lidx<LD, SW>((N - k) % N, cc)];
         V out;
         if ((rr & 1) == 0) {
             out.x = (zk.x + zm.x) * (T)0.5;
@@ -130,7 +131,7 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
         gstore_once(spec + ((size_t)z * N + y0 + rr) * H + k, out);
     }
     for (int rr = tid; rr < 2 * P; rr += NT) {
-        const V zh = lds[H * LD + (rr >> 1)];
+        const V zh = lds[lidx<LD, SW>(H, rr >> 1)];
         V out;
         out.x = (rr & 1) ? zh.y : zh.x;
         out.y = (T)0;
@@ -149,12 +150,12 @@ struct XInvArgs {
     const cplx_t<T>* tw;
     const DevScalars* S;
 };
-template <typename T, int N>
-__global__ __launch_bounds__((FftCfg<T, N>::PX * FftCfg<T, N>::TPC))
+template <typename T, int N, int P>
+__global__ __launch_bounds__((P * FftCfg<T, N>::TPC))
 void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
     using Cfg = FftCfg<T, N>;
-    constexpr int P = Cfg::PX, TPC = Cfg::TPC, H = N / 2, LD = P + 1;
+    constexpr int TPC = Cfg::TPC, H = N / 2, LD = P, SW = P - 1;
     constexpr int NT = P * TPC;
     __shared__ V lds[N * LD];
     __shared__ V stw[N];
@@ -185,35 +186,35 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
             const V An = gload_once(nyq + (size_t)z * N + y0 + 2 * c);
             const V Bn = gload_once(nyq + (size_t)z * N + y0 + 2 * c + 1);
             V zh; zh.x = An.x; zh.y = Bn.x;
-            lds[c] = z0;
-            lds[H * LD + c] = zh;
+            lds[lidx<LD, SW>(0, c)] = z0;
+            lds[lidx<LD, SW>(H, c)] = zh;
         } else {
             V zk; zk.x = A.x - B.y; zk.y = A.y + B.x;
             V zm; zm.x = A.x + B.y; zm.y = B.x - A.y;
-            lds[k * LD + c] = zk;
-            lds[(N - k) * LD + c] = zm;
+            lds[lidx<LD, SW>(k, c)] = zk;
+            lds[lidx<LD, SW>(N - k, c)] = zm;
         }
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, true>(lds, tid % P, tid / P, stw);
+    fft_stages<V, N, 1, P, TPC, LD, true, N, SW>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (N * 2 * P + NT - 1) / NT; r++) {
         const int e = tid + r * NT;
         if ((N * 2 * P) % NT != 0 && e >= N * 2 * P) break;
         const int x = e % N, rr = e / N;
-        const V zz = lds[x * LD + (rr >> 1)];
+        const V zz = lds[lidx<LD, SW>(x, rr >> 1)];
         st_once(&pres[idx3(x, y0 + rr, z, N)], (rr & 1) ? zz.y : zz.x);
     }
 }
 
 // ------------------------------------------------------------ column pass
 // C columns of length N, element (c, a) at data[base + a*sa + c*sc] with
-// base = blockIdx.x*bx + blockIdx.y*by.  MODE 0 forward, 1 inverse,
-// 2 forward + spectral divide + inverse (the z pass).  AXIS_CONTIG picks
-// the global access order (a fastest when the axis is contiguous).
-// NYQ marks the Nyquist plane, where the columns are y wavenumbers and
-// kx = n/2; otherwise the columns are kx wavenumbers and blockIdx.y is ky.
+// base = bxi*bx + byi*by.  MODE 0 forward, 1 inverse, 2 forward + spectral
+// divide + inverse (the z pass).  AXIS_CONTIG picks the global access order
+// (a fastest when the axis is contiguous).  NYQ marks the Nyquist plane,
+// where the columns are y wavenumbers and kx = n/2; otherwise the columns
+// are kx wavenumbers and byi is ky.
 template <typename T>
 struct ColArgs {
     cplx_t<T>* data;
@@ -223,40 +224,22 @@ struct ColArgs {
     double ntot;
     const DevScalars* S;
 };
-template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool NYQ, bool WF>
-__global__ __launch_bounds__((C * FftCfg<T, N>::TPC))
-void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
+// The columns of one workgroup; lds holds N (C + 1) values, stw N, skk N
+// (MODE 2).
+template <typename T, int N, int C, bool AXIS_CONTIG, int MODE, bool WF>
+__device__ __forceinline__ void cols_body(const ColArgs<T>& A, const int bxi, const int byi, const bool NYQ,
+                                          cplx_t<T>* __restrict__ lds, cplx_t<T>* __restrict__ stw,
+                                          double* __restrict__ skk) {
     using V = cplx_t<T>;
     constexpr int TPC = FftCfg<T, N>::TPC;
     constexpr int LD = AXIS_CONTIG ? C + 1 : C;
     constexpr int NT = C * TPC;
-    __shared__ V lds[N * LD];
-    __shared__ V stw[N];
-    const Blk blk = launch_blk(nbox);
-    ColArgs<T> A = one;
-    if (tab) A = tab[blk.box];
     MHIT_GLOBAL V* __restrict__ data = gmem(A.data);
     const size_t bx = A.bx, by = A.by, sa = A.sa, sc = A.sc;
     const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
     const MHIT_GLOBAL double* __restrict__ kk = gmem(A.kk);
     const double ntot = A.ntot;
-    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
-    if (S && S->halted) return;
     const int tid = threadIdx.x;
-    // Workgroups dispatch to the 8 XCDs round-robin, each XCD with its own
-    // L2.  When a row segment is half a 128-byte line, the two workgroups
-    // that share the line are renumbered to land on the same XCD, 8
-    // dispatches apart, so the second reads the line from L2.  (A batched
-    // launch stacks whole box grids of a multiple of 16 workgroups, so the
-    // pairing holds in it too.)
-    int bxi = blk.x, byi = blk.y;
-    if (!AXIS_CONTIG && C * (int)sizeof(V) == 64 && blk.nx % 2 == 0 && (blk.nx * blk.ny) % 16 == 0) {
-        const int L = blk.x + blk.nx * blk.y;
-        const int r = L % 16;
-        const int t = L - r + (r % 8) * 2 + r / 8;
-        bxi = t % blk.nx;
-        byi = t / blk.nx;
-    }
     MHIT_GLOBAL V* __restrict__ g = data + bxi * bx + byi * by;
     // whole 128-byte lines per row segment: stream the stores; a half-line
     // segment's other half is written by the paired workgroup
@@ -300,7 +283,6 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
             for (int r = 0; r < 8; r++) v[r] = ld(g + (size_t)(t + r * TPC) * sa + c * sc);
         }
         stage_twiddles<V, N, NT>(stw, tw, tid);
-        __shared__ double skk[MODE == 2 ? N : 1];
         if constexpr (MODE == 2) {
             for (int e = tid; e < N; e += NT) skk[e] = kk[e];
         }
@@ -415,7 +397,7 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
 #pragma unroll
         for (int r = 0; r < 8; r++) v[r] = ld(g + (size_t)(t + r * TPC) * sa + c * sc);
         stage_twiddles<V, N, NT>(stw, tw, tid);
-        __shared__ double skk[MODE == 2 ? N : 1];   // per-axis k^2, staged with the data
+        // per-axis k^2, staged with the data
         if constexpr (MODE == 2) {
             for (int e = tid; e < N; e += NT) skk[e] = kk[e];
         }
@@ -527,6 +509,50 @@ void k_fft_cols(ColArgs<T> one, const ColArgs<T>* __restrict__ tab, int nbox) {
     }
 }
 
+// One launch per column pass.  Workgroup rows y < N hold the bulk
+// spectrum's columns (AXIS_CONTIG, NYQ = false); the two rows y = N, N + 1
+// hold the N / C workgroups of the Nyquist plane (NYQ_AC, NYQ = true), with
+// their own arguments (`onen`, `tabn`), so the plane costs no launch of its
+// own.
+//
+// Workgroups dispatch to the 8 XCDs round-robin, each XCD with its own L2.
+// When a row segment is half a 128-byte line, the two workgroups that share
+// the line are renumbered to land on the same XCD, 8 dispatches apart, so
+// the second reads the line from L2: within each aligned group of 16
+// workgroups of one part (bulk or Nyquist), where the part starts on a
+// multiple of 16 in dispatch order.
+template <typename T, int N, int C, bool AC, int MODE, bool NYQ_AC, bool WF>
+__global__ __launch_bounds__((C * FftCfg<T, N>::TPC))
+void k_fft_cols(ColArgs<T> one, ColArgs<T> onen, const ColArgs<T>* __restrict__ tab,
+                const ColArgs<T>* __restrict__ tabn, int nbox) {
+    using V = cplx_t<T>;
+    __shared__ V lds[N * ((AC || NYQ_AC) ? C + 1 : C)];
+    __shared__ V stw[N];
+    __shared__ double skk[MODE == 2 ? N : 1];
+    const Blk blk = launch_blk(nbox);
+    const bool nyq = blk.y >= N;
+    ColArgs<T> A = nyq ? onen : one;
+    if (tab) A = nyq ? tabn[blk.box] : tab[blk.box];
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    if (S && S->halted) return;
+    const bool ac = nyq ? NYQ_AC : AC;
+    const int start = nyq ? blk.nx * N : 0;                  // first workgroup of the part
+    const int count = nyq ? 2 * blk.nx : blk.nx * N;
+    int L = blk.x + blk.nx * blk.y - start;
+    const long long first = (long long)blk.box * blk.nx * blk.ny + start;   // in dispatch order
+    if (!ac && C * (int)sizeof(V) == 64 && blk.nx % 2 == 0 && count % 16 == 0 && first % 16 == 0) {
+        const int r = L % 16;
+        L = L - r + (r % 8) * 2 + r / 8;
+    }
+    const int bxi = nyq ? L : L % blk.nx, byi = nyq ? 0 : L / blk.nx;
+    if constexpr (AC == NYQ_AC) {
+        cols_body<T, N, C, AC, MODE, WF>(A, bxi, byi, nyq, lds, stw, skk);   // one body for both parts
+    } else {
+        if (nyq) cols_body<T, N, C, NYQ_AC, MODE, WF>(A, bxi, byi, true, lds, stw, skk);
+        else cols_body<T, N, C, AC, MODE, WF>(A, bxi, byi, false, lds, stw, skk);
+    }
+}
+
 // ---------------------------------------------------------------- drivers
 // The argument of box 0 (`one`) and, for a batched launch (arena given),
 // the device table of all nb boxes.
@@ -550,9 +576,28 @@ struct PoissonPasses {
     using Cfg = FftCfg<T, N>;
     static constexpr int H = N / 2;
     using Boxes = mhit_ctx* const*;
+    // launch widths compiled for this size (fft_widths lists the same):
+    // the default first, alternatives from n = 64 up
+    static constexpr bool ALT = N >= 64;
+    static constexpr int Y0 = Cfg::CY, Y1 = Cfg::CY / 2;
+    static constexpr int Z0 = sizeof(T) == 4 ? Cfg::CY / 2 : Cfg::CY, Z1 = sizeof(T) == 4 ? Cfg::CY : Cfg::CY / 2;
+    static constexpr int X0 = Cfg::PX, X1 = Cfg::PX / 2, X2 = Cfg::PX * 2;
+    static constexpr bool X2OK = H % X2 == 0;
 
     static int x_fwd(Boxes bx, int nb, ArgArena* ar, hipStream_t st, const double* a, const double* b,
                      const double* w, int in) {
+        const int p = fft_width(bx[0], 2);
+        if constexpr (ALT) {
+            if (p == X1) return x_fwd_p<X1>(bx, nb, ar, st, a, b, w, in);
+            if constexpr (X2OK) {
+                if (p == X2) return x_fwd_p<X2>(bx, nb, ar, st, a, b, w, in);
+            }
+        }
+        return x_fwd_p<X0>(bx, nb, ar, st, a, b, w, in);
+    }
+    template <int P>
+    static int x_fwd_p(Boxes bx, int nb, ArgArena* ar, hipStream_t st, const double* a, const double* b,
+                       const double* w, int in) {
         XFwdArgs<T> one;
         const XFwdArgs<T>* tab;
         int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
@@ -569,85 +614,107 @@ struct PoissonPasses {
             return x;
         }, one, tab);
         if (rc) return rc;
-        dim3 grid(N / (2 * Cfg::PX), N, nb), block(Cfg::PX * Cfg::TPC);
-        if (in == X_IN_READY) k_fft_x_fwd<T, N, X_IN_READY><<<grid, block, 0, st>>>(one, tab, nb);
-        else if (in == X_IN_DIV) k_fft_x_fwd<T, N, X_IN_DIV><<<grid, block, 0, st>>>(one, tab, nb);
-        else k_fft_x_fwd<T, N, X_IN_FIELD><<<grid, block, 0, st>>>(one, tab, nb);
+        dim3 grid(N / (2 * P), N, nb), block(P * Cfg::TPC);
+        if (in == X_IN_READY) k_fft_x_fwd<T, N, X_IN_READY, P><<<grid, block, 0, st>>>(one, tab, nb);
+        else if (in == X_IN_DIV) k_fft_x_fwd<T, N, X_IN_DIV, P><<<grid, block, 0, st>>>(one, tab, nb);
+        else k_fft_x_fwd<T, N, X_IN_FIELD, P><<<grid, block, 0, st>>>(one, tab, nb);
         return MHIT_OK;
     }
     // A column pass in the context's transform form (Tuning::fft_wave):
     // the wavefront form where it fits, else the register-direct / LDS
-    // form.  Boxes of one batched launch share the form.
-    template <int C, bool AC, int MODE, bool NYQ>
-    static void launch_cols(const mhit_ctx* c, dim3 grid, const ColArgs<T>& one, const ColArgs<T>* tab,
-                            int nb, hipStream_t st) {
-        const dim3 block(C * Cfg::TPC);
+    // form.  Boxes of one batched launch share the form.  The grid is the
+    // bulk's (nxb, N) workgroups per box plus two rows for the Nyquist plane.
+    template <int C, bool AC, int MODE, bool NAC>
+    static void launch_cols(const mhit_ctx* c, int nxb, const ColArgs<T>& one, const ColArgs<T>& onen,
+                            const ColArgs<T>* tab, const ColArgs<T>* tabn, int nb, hipStream_t st) {
+        const dim3 grid(nxb, N + 2, nb), block(C * Cfg::TPC);
         if constexpr (wave_form_fits<T, N, C, AC>()) {
             if (c->tune.fft_wave) {
-                k_fft_cols<T, N, C, AC, MODE, NYQ, true><<<grid, block, 0, st>>>(one, tab, nb);
+                k_fft_cols<T, N, C, AC, MODE, NAC, true><<<grid, block, 0, st>>>(one, onen, tab, tabn, nb);
                 return;
             }
         }
-        k_fft_cols<T, N, C, AC, MODE, NYQ, false><<<grid, block, 0, st>>>(one, tab, nb);
+        k_fft_cols<T, N, C, AC, MODE, NAC, false><<<grid, block, 0, st>>>(one, onen, tab, tabn, nb);
     }
     template <int MODE>
     static int y_pass(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
-        constexpr int CY = Cfg::CY, CN = Cfg::CN;
-        ColArgs<T> one;
-        const ColArgs<T>* tab;
+        if constexpr (ALT) {
+            if (fft_width(bx[0], 0) == Y1) return y_pass_w<MODE, Y1>(bx, nb, ar, st);
+        }
+        return y_pass_w<MODE, Y0>(bx, nb, ar, st);
+    }
+    template <int MODE, int CY>
+    static int y_pass_w(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
+        ColArgs<T> one, onen;
+        const ColArgs<T> *tab, *tabn;
         // spec: columns kx at fixed z; axis y (stride H)
         int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return ColArgs<T>{(V*)c->spec, (size_t)CY, (size_t)N * H, (size_t)H, 1,
                               (const V*)c->tw, c->kk, 0.0, c->S};
         }, one, tab);
         if (rc) return rc;
-        launch_cols<CY, false, MODE, false>(bx[0], dim3(H / CY, N, nb), one, tab, nb, st);
         // nyq [z][y]: columns z, axis y contiguous
         rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
-            return ColArgs<T>{(V*)c->nyq, (size_t)CN * N, 0, 1, (size_t)N,
+            return ColArgs<T>{(V*)c->nyq, (size_t)CY * N, 0, 1, (size_t)N,
                               (const V*)c->tw, c->kk, 0.0, c->S};
-        }, one, tab);
+        }, onen, tabn);
         if (rc) return rc;
-        launch_cols<CN, true, MODE, true>(bx[0], dim3(N / CN, 1, nb), one, tab, nb, st);
+        launch_cols<CY, false, MODE, true>(bx[0], H / CY, one, onen, tab, tabn, nb, st);
         return MHIT_OK;
     }
+    // Default z width, FP32: half the y pass's columns per workgroup
+    // (64-byte row segments): the two FFTs of the z pass make it the
+    // longest-running column kernel, and smaller workgroups (two per CU at
+    // its register count) hide its barriers better (0.38 -> 0.33 ms at
+    // 512^3).  FP64 keeps 8 columns (128-byte segments; 4 columns measured
+    // 0.84 vs 0.55 ms).
     template <int MODE>
     static int z_pass(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
-        // FP32: half the y pass's columns per workgroup (64-byte row
-        // segments): the two FFTs of the z pass make it the longest-running
-        // column kernel, and smaller workgroups (two per CU at its register
-        // count) hide its barriers better (0.38 -> 0.33 ms at 512^3).  FP64
-        // keeps 8 columns (128-byte segments; 4 columns measured 0.84 vs
-        // 0.55 ms).
-        constexpr int CY = (sizeof(T) == 4) ? Cfg::CY / 2 : Cfg::CY;
+        if constexpr (ALT) {
+            if (fft_width(bx[0], 1) == Z1) return z_pass_w<MODE, Z1>(bx, nb, ar, st);
+        }
+        return z_pass_w<MODE, Z0>(bx, nb, ar, st);
+    }
+    template <int MODE, int CY>
+    static int z_pass_w(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
         const double ntot = (double)((long long)N * N * N);
-        ColArgs<T> one;
-        const ColArgs<T>* tab;
-        // spec: columns kx at fixed y (blockIdx.y); axis z (stride N*H)
+        ColArgs<T> one, onen;
+        const ColArgs<T> *tab, *tabn;
+        // spec: columns kx at fixed y; axis z (stride N*H)
         int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return ColArgs<T>{(V*)c->spec, (size_t)CY, (size_t)H, (size_t)N * H, 1,
                               (const V*)c->tw, c->kk, ntot, c->S};
         }, one, tab);
         if (rc) return rc;
-        launch_cols<CY, false, MODE, false>(bx[0], dim3(H / CY, N, nb), one, tab, nb, st);
         // nyq [z][y]: columns y, axis z (stride N)
         rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return ColArgs<T>{(V*)c->nyq, (size_t)CY, 0, (size_t)N, 1,
                               (const V*)c->tw, c->kk, ntot, c->S};
-        }, one, tab);
+        }, onen, tabn);
         if (rc) return rc;
-        launch_cols<CY, false, MODE, true>(bx[0], dim3(N / CY, 1, nb), one, tab, nb, st);
+        launch_cols<CY, false, MODE, false>(bx[0], H / CY, one, onen, tab, tabn, nb, st);
         return MHIT_OK;
     }
     static int x_inv(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
+        const int p = fft_width(bx[0], 2);
+        if constexpr (ALT) {
+            if (p == X1) return x_inv_p<X1>(bx, nb, ar, st);
+            if constexpr (X2OK) {
+                if (p == X2) return x_inv_p<X2>(bx, nb, ar, st);
+            }
+        }
+        return x_inv_p<X0>(bx, nb, ar, st);
+    }
+    template <int P>
+    static int x_inv_p(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
         XInvArgs<T> one;
         const XInvArgs<T>* tab;
         int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
             return XInvArgs<T>{(const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S};
         }, one, tab);
         if (rc) return rc;
-        dim3 grid(N / (2 * Cfg::PX), N, nb), block(Cfg::PX * Cfg::TPC);
-        k_fft_x_inv<T, N><<<grid, block, 0, st>>>(one, tab, nb);
+        dim3 grid(N / (2 * P), N, nb), block(P * Cfg::TPC);
+        k_fft_x_inv<T, N, P><<<grid, block, 0, st>>>(one, tab, nb);
         return MHIT_OK;
     }
     static int project(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
@@ -700,6 +767,36 @@ bool fft_size_supported(int n) {
         return true;
         default: return false;
     }
+}
+
+int fft_widths(int n, bool fp32, int pass, int out[3]) {
+    const int h = n / 2;
+    const int cy = pow2_divisor_cap(h, fp32 ? 16 : 8), px = pow2_divisor_cap(h, fp32 ? 8 : 4);
+    const bool alt = n >= 64;
+    int k = 0;
+    if (pass == 0) {
+        out[k++] = cy;
+        if (alt) out[k++] = cy / 2;
+    } else if (pass == 1) {
+        out[k++] = fp32 ? cy / 2 : cy;
+        if (alt) out[k++] = fp32 ? cy : cy / 2;
+    } else {
+        out[k++] = px;
+        if (alt) {
+            out[k++] = px / 2;
+            if (h % (2 * px) == 0) out[k++] = 2 * px;
+        }
+    }
+    return k;
+}
+
+int fft_width(const mhit_ctx* c, int pass) {
+    int w[3];
+    const int k = fft_widths(c->n, c->fp32, pass, w);
+    const int want = pass == 0 ? c->tune.fft_ycols : pass == 1 ? c->tune.fft_zcols : c->tune.fft_xrows;
+    for (int i = 0; i < k; i++)
+        if (w[i] == want) return want;
+    return w[0];
 }
 
 enum class Op { Project, ProjectReady, Solve, R2C, C2R };
@@ -758,9 +855,10 @@ int poisson_project_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipSt
     mhit_ctx* c = boxes[0];
     for (int b = 0; b < nb; b++)
         if (boxes[b]->n != c->n || boxes[b]->fp32 != c->fp32 || !boxes[b]->march ||
-            boxes[b]->tune.fft_wave != c->tune.fft_wave) {
-            set_error("batched projection: boxes differ in size, precision or transform form, or do not "
-                      "use the fused passes");
+            boxes[b]->tune.fft_wave != c->tune.fft_wave || fft_width(boxes[b], 0) != fft_width(c, 0) ||
+            fft_width(boxes[b], 1) != fft_width(c, 1) || fft_width(boxes[b], 2) != fft_width(c, 2)) {
+            set_error("batched projection: boxes differ in size, precision, transform form or launch "
+                      "widths, or do not use the fused passes");
             return MHIT_ERR_ARG;
         }
     return c->fp32 ? dispatch_batch<float>(boxes, nb, arena, st) : dispatch_batch<double>(boxes, nb, arena, st);

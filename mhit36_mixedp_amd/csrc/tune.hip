@@ -11,9 +11,15 @@
 // period (sort_every) is settable but not tuned: its pay-off depends on how
 // far tracers move, which a short trial does not see.
 //
-// The column transforms' form (fft_wave) is tuned as well: the wavefront
+// The FFT passes are tuned too: the column transforms' form (fft_wave)
+// and the launch widths (columns per workgroup of the y and z passes, row
+// pairs per workgroup of the x passes; each width a compiled kernel, so
+// the choice also sets the launch bounds: 64 threads per unit of width and
+// 1/8 of the transform length, and the LDS per workgroup).  The wavefront
 // form and the register-direct form compute the same DFT in a different
-// operation order, so that choice changes results at the rounding level.
+// operation order, so that choice changes results at the rounding level;
+// a width changes the form only where it moves a pass to or from eight
+// columns (the wavefront form's width), and otherwise no bit.
 //
 // mhit_autotune times candidate values one parameter at a time on the
 // context's own state, with the production path (captured graphs), and
@@ -147,7 +153,8 @@ int time_steps(mhit_ctx* c, int steps, double* ms) {
     return MHIT_OK;
 }
 
-void to_public(const Tuning& t, mhit_tuning* out, double ms) {
+void to_public(const mhit_ctx* c, mhit_tuning* out, double ms) {
+    const Tuning& t = c->tune;
     out->ns_mz = t.ns_mz;
     out->ac_mz = t.ac_mz;
     out->xcd = t.xcd;
@@ -155,13 +162,27 @@ void to_public(const Tuning& t, mhit_tuning* out, double ms) {
     out->ns_my = t.ns_my;
     out->ac_my = t.ac_my;
     out->fft_wave = t.fft_wave;
+    out->fft_ycols = fft_width(c, 0);
+    out->fft_zcols = fft_width(c, 1);
+    out->fft_xrows = fft_width(c, 2);
     out->ms_per_step = ms;
 }
 
-bool valid(const mhit_tuning* t) {
+// 0 (the default) or a width compiled for the context's size and precision
+bool valid_width(const mhit_ctx* c, int pass, int w) {
+    if (w == 0) return true;
+    int ws[3];
+    const int k = fft_widths(c->n, c->fp32, pass, ws);
+    for (int i = 0; i < k; i++)
+        if (ws[i] == w) return true;
+    return false;
+}
+
+bool valid(const mhit_ctx* c, const mhit_tuning* t) {
     auto rows = [](int r) { return r == 8 || r == 16; };
     return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
-           rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1);
+           rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1) &&
+           valid_width(c, 0, t->fft_ycols) && valid_width(c, 1, t->fft_zcols) && valid_width(c, 2, t->fft_xrows);
 }
 
 }  // namespace
@@ -170,12 +191,12 @@ extern "C" {
 
 int mhit_get_tuning(mhit_ctx* c, mhit_tuning* out) {
     if (!c || !out) { set_error("null argument"); return MHIT_ERR_ARG; }
-    to_public(c->tune, out, 0.0);
+    to_public(c, out, 0.0);
     return MHIT_OK;
 }
 
 int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
-    if (!c || !t || !valid(t)) { set_error("bad tuning"); return MHIT_ERR_ARG; }
+    if (!c || !t || !valid(c, t)) { set_error("bad tuning"); return MHIT_ERR_ARG; }
     c->tune.ns_mz = t->ns_mz;
     c->tune.ac_mz = t->ac_mz;
     c->tune.xcd = t->xcd;
@@ -183,6 +204,9 @@ int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
     c->tune.ns_my = t->ns_my;
     c->tune.ac_my = t->ac_my;
     c->tune.fft_wave = t->fft_wave;
+    c->tune.fft_ycols = t->fft_ycols;
+    c->tune.fft_zcols = t->fft_zcols;
+    c->tune.fft_xrows = t->fft_xrows;
     clear_graphs(c);
     return MHIT_OK;
 }
@@ -204,6 +228,11 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
         if (v.empty()) v.push_back(n);
         return v;
     };
+    auto widths = [&](int pass) {
+        int w[3];
+        const int k = fft_widths(n, c->fp32, pass, w);
+        return std::vector<int>(w, w + k);
+    };
     struct Param { int* field; std::vector<int> cand; bool active; };
     std::vector<Param> params = {
         {&c->tune.ns_my, {16, 8}, true},
@@ -212,6 +241,9 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
         {&c->tune.ac_mz, zlist({4, 8, 16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
         {&c->tune.fft_wave, {1, 0}, true},
+        {&c->tune.fft_ycols, widths(0), widths(0).size() > 1},
+        {&c->tune.fft_zcols, widths(1), widths(1).size() > 1},
+        {&c->tune.fft_xrows, widths(2), widths(2).size() > 1},
     };
     double best_ms = 0.0;
     rc = MHIT_OK;
@@ -239,7 +271,7 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
     release(snap);
     clear_graphs(c);
     if (rc) return rc;
-    if (out) to_public(c->tune, out, best_ms);
+    if (out) to_public(c, out, best_ms);
     return MHIT_OK;
 }
 

@@ -64,6 +64,8 @@ int main(void) {
   printf("mhit_params %zu\n", sizeof(mhit_params));
   printf("mhit_diag %zu\n", sizeof(mhit_diag));
   printf("mhit_profile %zu\n", sizeof(mhit_profile));
+  printf("mhit_tuning %zu\n", sizeof(mhit_tuning));
+  F(mhit_tuning, fft_wave) F(mhit_tuning, fft_xrows) F(mhit_tuning, ms_per_step)
   F(mhit_params, dt) F(mhit_params, npart) F(mhit_params, nx) F(mhit_params, cfl_abort)
   F(mhit_diag, ke) F(mhit_diag, nan_count) F(mhit_profile, ms) F(mhit_profile, bytes_per_call)
   return 0;
@@ -79,10 +81,12 @@ int main(void) {
     assert int(out["mhit_params"]) == ctypes.sizeof(_abi.Params)
     assert int(out["mhit_diag"]) == ctypes.sizeof(_abi.Diag)
     assert int(out["mhit_profile"]) == ctypes.sizeof(_abi.Profile)
+    assert int(out["mhit_tuning"]) == ctypes.sizeof(_abi.Tuning)
     for key, val in out.items():
         if "." in key:
             t, m = key.split(".")
-            cls = {"mhit_params": _abi.Params, "mhit_diag": _abi.Diag, "mhit_profile": _abi.Profile}[t]
+            cls = {"mhit_params": _abi.Params, "mhit_diag": _abi.Diag, "mhit_profile": _abi.Profile,
+                   "mhit_tuning": _abi.Tuning}[t]
             assert getattr(cls, m).offset == int(val), key
 
 

@@ -145,3 +145,39 @@ def test_both_transform_forms_match_numpy(n, prec, form):
     ph[0, 0, 0] = 0.0
     ref = np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2))
     assert relerr(p, ref) < (1e-12 if prec == 0 else 1e-5)
+
+
+@pytest.mark.parametrize("form", [0, 1], ids=["direct", "wavefront"])
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [64, 96, 256])
+def test_every_launch_width_matches_numpy(n, prec, form):
+    """The alternative FFT launch widths (Tuning fft_ycols / fft_zcols /
+    fft_xrows, one at a time) against numpy: forward, inverse and the
+    Poisson solve, in both transform forms (eight columns take the
+    wavefront form when it is on)."""
+    print(f"[fft widths] n={n} prec={prec} form={form}", flush=True)
+    rng = np.random.default_rng(700 + n + form)
+    x = rng.standard_normal((n, n, n))
+    tol = 1e-12 if prec == 0 else 2e-6
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kz, ky, kx = np.meshgrid(k, k, k[: n // 2 + 1], indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k2[0, 0, 0] = 1.0
+    ph = -np.fft.rfftn(x) / k2
+    ph[0, 0, 0] = 0.0
+    ref = np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2))
+    with _solver(n, prec) as s:
+        base = s.tuning()
+        alts = [dict(fft_ycols=base["fft_ycols"] // 2),
+                dict(fft_zcols=base["fft_zcols"] * 2 if prec == 1 else base["fft_zcols"] // 2),
+                dict(fft_xrows=base["fft_xrows"] // 2)]
+        if (n // 2) % (2 * base["fft_xrows"]) == 0:
+            alts.append(dict(fft_xrows=base["fft_xrows"] * 2))
+        for alt in alts:
+            kw = dict(fft_wave=form, fft_ycols=0, fft_zcols=0, fft_xrows=0)
+            kw.update(alt)
+            s.set_tuning(**kw)
+            assert all(s.tuning()[kk] == v for kk, v in alt.items())
+            assert relerr(s.fft_r2c(x), np.fft.rfftn(x)) < tol, alt
+            assert relerr(s.fft_c2r(np.fft.rfftn(x)), x * n ** 3) < tol, alt
+            assert relerr(s.poisson_solve(x), ref) < (1e-12 if prec == 0 else 1e-5), alt

@@ -24,9 +24,11 @@ def test_autotune_restores_state_and_results_are_shape_independent():
             s.step(2)
         t = a.autotune(steps=2)
         print(t, flush=True)
-        # the transform form changes rounding (and is a no-op at 48^3); the
+        # the transform form changes rounding (and is a no-op at 48^3; so
+        # are the FFT widths, compiled with alternatives from 64 up); the
         # launch shapes change no bit
-        b.set_tuning(fft_wave=t["fft_wave"])
+        b.set_tuning(fft_wave=t["fft_wave"], fft_ycols=t["fft_ycols"], fft_zcols=t["fft_zcols"],
+                     fft_xrows=t["fft_xrows"])
         assert t["ms_per_step"] > 0
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
         assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16) and t["fft_wave"] in (0, 1)
@@ -53,6 +55,9 @@ def test_autotune_restores_state_and_results_are_shape_independent():
     (48, dict(ns_mz=4, ac_mz=4, xcd=1, ns_my=16, ac_my=16)),
     # 24 rows: the 16-row tiles end in a partial tile (the default is 8)
     (24, dict(ns_mz=8, ac_mz=8, xcd=0, ns_my=16, ac_my=16)),
+    # x passes: no wavefront form, so every row-pair width is bit-neutral
+    (64, dict(fft_xrows=4)),
+    (96, dict(fft_xrows=16)),
 ])
 def test_set_tuning_is_bitwise_neutral(n, shape):
     from mhit36_mixedp_amd import Solver
@@ -86,3 +91,48 @@ def test_mhit_tune_overrides_and_ignores_bad_values(monkeypatch):
         t = s.tuning()
     for k in ("ns_mz", "ac_mz", "ns_my", "ac_my", "xcd", "sort_every"):
         assert t[k] == base[k], (k, t[k], base[k])
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [64, 96, 128])
+def test_fft_widths_are_bitwise_neutral_in_one_form(n, prec):
+    """Every compiled FFT launch width gives the same bits as the default
+    when both use the register-direct form (the wavefront form exists only
+    at eight columns)."""
+    from mhit36_mixedp_amd import Solver
+    print(f"[tuning] FFT widths n={n} prec={prec}", flush=True)
+    kw = dict(nx=n, phiflag=1, sigma=0.5, dt=5e-4, poisson_precision=prec)
+    with Solver(**kw) as a, Solver(**kw) as b:
+        base = b.tuning()
+        alt = dict(fft_ycols=base["fft_ycols"] // 2,
+                   fft_zcols=base["fft_zcols"] * 2 if prec == 1 else base["fft_zcols"] // 2,
+                   fft_xrows=base["fft_xrows"] // 2)
+        a.set_tuning(fft_wave=0, **alt)
+        b.set_tuning(fft_wave=0)
+        t = a.tuning()
+        assert all(t[k] == v for k, v in alt.items()), (t, alt)
+        for s in (a, b):
+            s.init_random_velocity(5, kmax=4.0, urms=1.0)
+            s.init_drops(2, 0.7, 5)
+            s.step(4)
+        for f in (1, 2, 3, 4, 5):
+            assert np.array_equal(a.download_field(f), b.download_field(f)), f
+
+
+def test_fft_width_validation():
+    """Widths that are not compiled for the size are rejected by
+    mhit_set_tuning; 0 restores the default."""
+    from mhit36_mixedp_amd import MhitError, Solver
+    with Solver(nx=64, poisson_precision=1) as s:
+        base = s.tuning()
+        assert (base["fft_ycols"], base["fft_zcols"], base["fft_xrows"]) == (16, 8, 8)
+        for bad in (dict(fft_ycols=32), dict(fft_zcols=4), dict(fft_xrows=3)):
+            with pytest.raises(MhitError):
+                s.set_tuning(**bad)
+        s.set_tuning(fft_ycols=8)
+        assert s.tuning()["fft_ycols"] == 8
+        s.set_tuning(fft_ycols=0)
+        assert s.tuning()["fft_ycols"] == 16
+    with Solver(nx=32, poisson_precision=1) as s:   # no alternatives below 64
+        with pytest.raises(MhitError):
+            s.set_tuning(fft_xrows=4)
