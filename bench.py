@@ -222,13 +222,27 @@ def ensemble_leg(args, batched_only=False):
         ens.step(args.ensemble_steps)
         t1 = time.perf_counter()
         together = (t1 - t0) / args.ensemble_steps          # s per ensemble step
+        # the batched shapes tuned per grid size on the live state (restored),
+        # then timed like the default shapes
+        ens.autotune(steps=3)
+        tuned_shapes = {f"{n}^3": ens.boxes[16 * k].tuning() for k, n in enumerate(ENSEMBLE_SIZES)}
+        for t in tuned_shapes.values():
+            t.pop("ms_per_step", None)
+            t.pop("sort_every", None)
+        ens.step(args.warmup)
+        t0 = time.perf_counter()
+        ens.step(args.ensemble_steps)
+        t1 = time.perf_counter()
+        tuned = (t1 - t0) / args.ensemble_steps
         if batched_only:
             nbox = len(params)
             return {"workload": "16x64^3 + 16x96^3 + 16x128^3 boxes, half multiphase, FP32 Poisson",
                     "boxes": nbox, "cells": cells, "steps": args.ensemble_steps,
                     "box_steps_per_s": round(nbox / together, 1),
                     "ms_per_ensemble_step": round(together * 1e3, 4),
-                    "hbm_GBps": round(nbytes / together / 1e9, 1)}
+                    "hbm_GBps": round(nbytes / together / 1e9, 1),
+                    "autotuned_box_steps_per_s": round(nbox / tuned, 1),
+                    "autotuned_shapes": tuned_shapes}
         ens.set_batched(False)
         ens.step(args.warmup)
         t0 = time.perf_counter()
@@ -250,6 +264,8 @@ def ensemble_leg(args, batched_only=False):
         "box_steps_per_s": round(nbox / together, 1),
         "ms_per_ensemble_step": round(together * 1e3, 4),
         "hbm_GBps": round(nbytes / together / 1e9, 1),
+        "autotuned_box_steps_per_s": round(nbox / tuned, 1),
+        "autotuned_shapes": tuned_shapes,
         "branch_per_box_box_steps_per_s": round(nbox / branches, 1),
         "one_at_a_time_box_steps_per_s": round(nbox / alone, 1),
         "one_size_box_steps_per_s": one_size_rates(args),
@@ -347,7 +363,8 @@ def main():
     ens = None
     if not args.no_ensemble:
         ens = ensemble_leg(args)
-        log(f"[bench] ensemble: {ens['box_steps_per_s']} box-steps/s together, "
+        log(f"[bench] ensemble: {ens['box_steps_per_s']} box-steps/s together "
+            f"({ens['autotuned_box_steps_per_s']} autotuned), "
             f"{ens['one_at_a_time_box_steps_per_s']} one box at a time, {ens['hbm_GBps']} GB/s")
 
     # --- per-stage table and roofline of the dominant kernel ----------------------

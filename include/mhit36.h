@@ -329,6 +329,17 @@ int mhit_ensemble_create(int32_t nbox, const mhit_params* params, mhit_ensemble*
 int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);
 int mhit_ensemble_step(mhit_ensemble* ens, int nsteps);
 int mhit_ensemble_set_batched(mhit_ensemble* ens, int32_t on);
+/* Batched form: tune the launch shapes per grid size and Poisson precision
+ * (z chunk of the fused passes, their tile rows, the FFT transform form and
+ * launch widths; the same value for every box of the class), one setting
+ * at a time, each candidate timed over `steps` ensemble steps on the live
+ * state; keeps the fastest and restores every box's state (as
+ * mhit_autotune does for one box; the reference's analogue is cuDecomp's
+ * start-up autotune, multi/main.f90:80-136).  *ms_per_step returns the
+ * ensemble step time with the chosen settings, which each box's
+ * mhit_get_tuning reports.  The transform form changes rounding only;
+ * mhit_ensemble_set_batched resets the shapes.                            */
+int mhit_ensemble_autotune(mhit_ensemble* ens, int steps, double* ms_per_step);
 int mhit_ensemble_destroy(mhit_ensemble* ens);
 
 /* ---- per-stage timing ------------------------------------------------ */

@@ -221,6 +221,7 @@ EXPORTS = [
     ("mhit_ensemble_box", ctypes.c_int, [_vp, ctypes.c_int32, _P(_vp)]),
     ("mhit_ensemble_step", ctypes.c_int, [_vp, ctypes.c_int]),
     ("mhit_ensemble_set_batched", ctypes.c_int, [_vp, ctypes.c_int32]),
+    ("mhit_ensemble_autotune", ctypes.c_int, [_vp, ctypes.c_int, _P(ctypes.c_double)]),
     ("mhit_ensemble_destroy", ctypes.c_int, [_vp]),
     ("mhit_version", ctypes.c_char_p, []),
 ]

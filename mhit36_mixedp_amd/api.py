@@ -332,6 +332,15 @@ class Ensemble:
     def step(self, nsteps: int = 1):
         _check(self.lib.mhit_ensemble_step(self._h, int(nsteps)), "mhit_ensemble_step")
 
+    def autotune(self, steps: int = 3) -> float:
+        """Tune the batched launch shapes per grid size on the current
+        state (restored afterwards); returns ms per ensemble step with the
+        chosen settings (each box's ``tuning()`` reports them)."""
+        ms = ctypes.c_double()
+        _check(self.lib.mhit_ensemble_autotune(self._h, int(steps), ctypes.byref(ms)),
+               "mhit_ensemble_autotune")
+        return ms.value
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             for b in self.boxes:

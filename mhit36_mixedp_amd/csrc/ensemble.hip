@@ -8,7 +8,7 @@
 // timestep is a few dozen launches instead of ten per box.  Unbatched, the
 // branches are the boxes' own timesteps (the capture forks from the
 // ensemble stream into every box's stream and joins back).  The tracer
-// re-ordering (hipCUB radix sort, every sort_every steps) runs eagerly on
+// re-ordering (a radix sort, every sort_every steps) runs eagerly on
 // the ensemble stream before the replay.  Graphs are keyed by every box's
 // current buffers (the ping-pong parities, tracer columns).  Boxes share
 // nothing; a box stopped by the CFL check stays frozen (its kernels return
@@ -428,6 +428,133 @@ int mhit_ensemble_set_batched(mhit_ensemble* e, int32_t on) {
         if (!e->batched) default_tuning(c, false);
         clear_graphs(c);
     }
+    return MHIT_OK;
+}
+
+// ms per ensemble step over `steps` replays after one untimed step (the
+// graph capture); the caller restores the boxes' states afterwards.
+static int time_ensemble(mhit_ensemble* e, int steps, double* ms) {
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+    clear_ensemble_graphs(e);
+    for (mhit_ctx* c : e->boxes) clear_graphs(c);   // launch shapes are baked into graphs
+    int rc = mhit_ensemble_step(e, 1);
+    if (rc) return rc;
+    hipEvent_t a = nullptr, b = nullptr;
+    MHIT_HIP_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        set_error("ensemble autotune: event creation failed");
+        return MHIT_ERR_HIP;
+    }
+    hipError_t err = hipEventRecord(a, e->stream);
+    for (int s = 0; s < steps && err == hipSuccess && rc == MHIT_OK; s++) rc = ensemble_graph_step(e);
+    if (err == hipSuccess && rc == MHIT_OK) err = hipEventRecord(b, e->stream);
+    if (err == hipSuccess && rc == MHIT_OK) err = hipEventSynchronize(b);
+    float f = 0.0f;
+    if (err == hipSuccess && rc == MHIT_OK) err = hipEventElapsedTime(&f, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    MHIT_HIP_TRY(err);
+    int status = MHIT_OK;
+    if ((rc = finish_ensemble(e, &status))) return rc;
+    if (status != MHIT_OK) {
+        set_error("ensemble autotune: a trial stopped on cou > 7");
+        return status;
+    }
+    *ms = (double)f / steps;
+    return MHIT_OK;
+}
+
+int mhit_ensemble_autotune(mhit_ensemble* e, int steps, double* ms_per_step) {
+    if (!e || steps <= 0) { set_error("bad arguments"); return MHIT_ERR_ARG; }
+    if (!e->batched || !e->use_graph || !can_batch(e)) {
+        set_error("ensemble autotune: batched graph mode with the fused passes only");
+        return MHIT_ERR_STATE;
+    }
+    for (mhit_ctx* c : e->boxes)
+        if (c->prof) { set_error("ensemble autotune: disable profiling first"); return MHIT_ERR_STATE; }
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+    for (mhit_ctx* c : e->boxes)
+        if (c->S_host.halted) { set_error("stopped: cou > 7"); return MHIT_ERR_CFL; }
+    const size_t nb = e->boxes.size();
+    std::vector<StateCopy*> snaps(nb, nullptr);
+    int rc = MHIT_OK;
+    for (size_t b = 0; b < nb && rc == MHIT_OK; b++) rc = state_copy_take(e->boxes[b], &snaps[b]);
+    auto restore_all = [&]() {
+        int r = MHIT_OK;
+        for (size_t b = 0; b < nb; b++) {
+            const int q = state_copy_restore(e->boxes[b], snaps[b]);
+            if (r == MHIT_OK) r = q;
+        }
+        return r;
+    };
+    auto trial = [&](double* ms) {
+        int r = time_ensemble(e, steps, ms);
+        const int q = restore_all();
+        return r != MHIT_OK ? r : q;
+    };
+    double best_ms = 0.0;
+    if (rc == MHIT_OK) rc = trial(&best_ms);
+    // one size and precision at a time, one setting at a time (the same
+    // value for every box of the class), keeping the fastest
+    const auto classes = group_by(e->boxes, [](mhit_ctx* c) { return std::make_pair(c->n, c->fp32); });
+    for (const auto& cls : classes) {
+        if (rc != MHIT_OK) break;
+        mhit_ctx* c0 = cls[0];
+        const int n = c0->n;
+        std::vector<int> zc = {c0->tune.ns_mz};
+        for (int z = n; z >= 8; z /= 2)
+            if (z != c0->tune.ns_mz) zc.push_back(z);
+        auto widths = [&](int pass) {
+            int w[3];
+            const int k = fft_widths(n, c0->fp32, pass, w);
+            return std::vector<int>(w, w + k);
+        };
+        struct Param {
+            int Tuning::*field;
+            int Tuning::*also;   // set together (the two fused passes' z chunk)
+            std::vector<int> cand;
+        };
+        const std::vector<Param> params = {
+            {&Tuning::ns_mz, &Tuning::ac_mz, zc},
+            {&Tuning::ns_my, nullptr, {16, 8}},
+            {&Tuning::ac_my, nullptr, {16, 8}},
+            {&Tuning::fft_wave, nullptr, {1, 0}},
+            {&Tuning::fft_ycols, nullptr, widths(0)},
+            {&Tuning::fft_zcols, nullptr, widths(1)},
+            {&Tuning::fft_xrows, nullptr, widths(2)},
+        };
+        for (const Param& p : params) {
+            if (rc != MHIT_OK) break;
+            if (p.cand.size() < 2) continue;
+            const int keep = c0->tune.*(p.field);
+            int best = keep;
+            double bms = best_ms;   // the current setting's time
+            for (int v : p.cand) {
+                if (v == keep) continue;
+                for (mhit_ctx* c : cls) {
+                    c->tune.*(p.field) = v;
+                    if (p.also) c->tune.*(p.also) = v;
+                }
+                double ms = 0.0;
+                if ((rc = trial(&ms)) != MHIT_OK) break;
+                if (ms < bms) { bms = ms; best = v; }
+            }
+            for (mhit_ctx* c : cls) {
+                c->tune.*(p.field) = best;
+                if (p.also) c->tune.*(p.also) = best;
+            }
+            best_ms = bms;
+        }
+    }
+    if (rc == MHIT_OK) rc = trial(&best_ms);
+    for (StateCopy* s : snaps) state_copy_free(s);
+    MHIT_HIP_TRY(hipStreamSynchronize(e->stream));
+    clear_ensemble_graphs(e);
+    for (mhit_ctx* c : e->boxes) clear_graphs(c);
+    if (rc != MHIT_OK) return rc;
+    if (ms_per_step) *ms_per_step = best_ms;
     return MHIT_OK;
 }
 

@@ -217,6 +217,13 @@ int fft_widths(int n, bool fp32, int pass, int out[3]);
 // default.
 int fft_width(const mhit_ctx* c, int pass);
 
+// tune.hip: a device copy of everything a timestep reads (restored into
+// the current owners of the buffer roles) and the scalars
+struct StateCopy;
+int state_copy_take(mhit_ctx* c, StateCopy** out);
+int state_copy_restore(mhit_ctx* c, const StateCopy* s);
+void state_copy_free(StateCopy* s);
+
 // poisson.hip
 enum class XSrc { Divergence, Real };
 int poisson_project(mhit_ctx* c);                       // u*,v*,w* -> pres

@@ -187,6 +187,26 @@ bool valid(const mhit_ctx* c, const mhit_tuning* t) {
 
 }  // namespace
 
+namespace mhit {
+// A context's state copy for other tuners (the ensemble's).
+struct StateCopy {
+    Snapshot snap;
+};
+int state_copy_take(mhit_ctx* c, StateCopy** out) {
+    StateCopy* s = new StateCopy();
+    const int rc = take(c, s->snap);
+    if (rc) { delete s; return rc; }
+    *out = s;
+    return MHIT_OK;
+}
+int state_copy_restore(mhit_ctx* c, const StateCopy* s) { return restore(c, s->snap); }
+void state_copy_free(StateCopy* s) {
+    if (!s) return;
+    release(s->snap);
+    delete s;
+}
+}  // namespace mhit
+
 extern "C" {
 
 int mhit_get_tuning(mhit_ctx* c, mhit_tuning* out) {

@@ -145,3 +145,52 @@ def test_batched_shapes_fill_the_gpu():
         # halving stops at the first workgroup count that reaches the
         # target: the same count for 16 boxes and for 8
         assert z8 * 2 == z16 and w8 == w16
+
+
+def test_ensemble_autotune_restores_state_and_matches_solo():
+    """mhit_ensemble_autotune leaves every box's state as it found it, and
+    the tuned ensemble still equals each box stepped alone with the same
+    transform form and FFT widths (the launch shapes change no bit)."""
+    from mhit36_mixedp_amd import Ensemble, Solver
+    kws = [dict(nx=64, phiflag=1, partflag=1, sigma=0.5), dict(nx=64, phiflag=0, partflag=0),
+           dict(nx=96, phiflag=1, partflag=0, sigma=0.5), dict(nx=96, phiflag=0, partflag=1)]
+    print("[ensemble] autotune, 2 x 64^3 + 2 x 96^3", flush=True)
+
+    def fields(kw):
+        return (1, 2, 3, 4, 5, 6, 7, 8, 9) if kw["phiflag"] else (1, 2, 3, 4, 6, 7, 8)
+
+    def state(s, kw):
+        st = {f: s.download_field(f) for f in fields(kw)}
+        if kw["partflag"]:
+            st.update({"t_" + k: v for k, v in s.download_tracers().items()})
+        return st
+
+    with Ensemble([_params(kw) for kw in kws]) as ens:
+        solos = [Solver(_params(kw)) for kw in kws]
+        for i, kw in enumerate(kws):
+            _init(ens.boxes[i], 40 + i, kw)
+            _init(solos[i], 40 + i, kw)
+        ens.step(2)
+        before = [state(b, kw) for b, kw in zip(ens.boxes, kws)]
+        ms = ens.autotune(steps=2)
+        assert ms > 0
+        for i, kw in enumerate(kws):
+            after = state(ens.boxes[i], kw)
+            for k in before[i]:
+                assert np.array_equal(before[i][k], after[k]), (i, k)
+        for i, s in enumerate(solos):
+            t = ens.boxes[i].tuning()
+            print(i, t, flush=True)
+            s.step(2)
+            s.set_tuning(fft_wave=t["fft_wave"], fft_ycols=t["fft_ycols"], fft_zcols=t["fft_zcols"],
+                         fft_xrows=t["fft_xrows"])
+        ens.step(3)
+        for s in solos:
+            s.step(3)
+        for i, kw in enumerate(kws):
+            a, b = state(ens.boxes[i], kw), state(solos[i], kw)
+            for k in a:
+                assert np.array_equal(a[k], b[k]), (i, k)
+            assert ens.boxes[i].diag() == solos[i].diag(), i
+        for s in solos:
+            s.close()
