@@ -216,27 +216,40 @@ __device__ __forceinline__ double rsq_refined(double x) {
 // emulated exactly and the hardware reciprocal estimate anywhere within
 // 2 ulp of 1/(ma + mb)).  a <= 0 gives -inf (a = 0)
 // or NaN, as log(a/b) would.
+// fma(a, b, c) in FP64 as the three-address VOP3 instruction (the same
+// IEEE fused multiply-add).  Left to itself the compiler emits the
+// two-address v_fmac_f64 and, when the addend must survive (a polynomial
+// coefficient held across a loop), first copies it into the destination.
+__device__ __forceinline__ double fma3(double a, double b, double c) {
+    double r;
+    asm("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 __device__ __forceinline__ double ln_ratio(double a, double b) {
     constexpr double LN2_HI = 0x1.62e42fee00000p-1;   // ln 2 to 32 bits
     constexpr double LN2_LO = 0x1.a39ef35793c76p-33;  // ln 2 - LN2_HI
     int ea, eb;
     double ma = frexp(a, &ea);
     const double mb = frexp(b, &eb);
-    if (ma < mb * 0.70710678118654752440) { ma = ma * 2.0; ea -= 1; }
-    else if (ma >= mb * 1.41421356237309504880) { ma = ma * 0.5; ea += 1; }
+    // q = ma/mb into [sqrt(1/2), sqrt(2)): a power-of-two rescale of ma
+    // (exact), selected without a branch
+    const int d = (ma < mb * 0.70710678118654752440) ? 1 : ((ma >= mb * 1.41421356237309504880) ? -1 : 0);
+    ma = ldexp(ma, d);
+    ea -= d;
     const double s = (ma - mb) * rcp_refined(ma + mb);
     const double t = s * s;
     double p = 1.0 / 23.0;
-    p = fma(p, t, 1.0 / 21.0);
-    p = fma(p, t, 1.0 / 19.0);
-    p = fma(p, t, 1.0 / 17.0);
-    p = fma(p, t, 1.0 / 15.0);
-    p = fma(p, t, 1.0 / 13.0);
-    p = fma(p, t, 1.0 / 11.0);
-    p = fma(p, t, 1.0 / 9.0);
-    p = fma(p, t, 1.0 / 7.0);
-    p = fma(p, t, 1.0 / 5.0);
-    p = fma(p, t, 1.0 / 3.0);
+    p = fma3(p, t, 1.0 / 21.0);
+    p = fma3(p, t, 1.0 / 19.0);
+    p = fma3(p, t, 1.0 / 17.0);
+    p = fma3(p, t, 1.0 / 15.0);
+    p = fma3(p, t, 1.0 / 13.0);
+    p = fma3(p, t, 1.0 / 11.0);
+    p = fma3(p, t, 1.0 / 9.0);
+    p = fma3(p, t, 1.0 / 7.0);
+    p = fma3(p, t, 1.0 / 5.0);
+    p = fma3(p, t, 1.0 / 3.0);
     const double s2 = 2.0 * s;
     const double E = (double)(ea - eb);
     double r = fma(E, LN2_HI, fma(E, LN2_LO, fma(s2 * t, p, s2)));

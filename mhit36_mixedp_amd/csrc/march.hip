@@ -460,16 +460,20 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         flux(k, tx, ty, fx, fy, fz);
         sFx[bf][ty * (MX + 2) + tx + 1] = fx;
         sFy[bf][(ty + 1) * MX + tx] = fy;
-        // one edge point per thread of waves 0-2 (no wave takes both
-        // branches): y = -1 and y = MY rows on threads [0, 2 MX), x = -1
-        // and x = MX columns on [2 MX, 2 MX + 2 MY)
-        if (tid < 2 * MX) {
-            const int a = tid >> 1, b = (tid & 1) ? MY : -1;
+        // one edge point per thread, on waves that do not take the second
+        // psi point of the commit (threads [0, 336) at 64 x 16), so that
+        // the extra work spreads over the SIMDs: y = -1 and y = MY rows on
+        // threads [EY, EY + 2 MX), x = -1 and x = MX columns on
+        // [EX, EX + 2 MY)
+        constexpr int EY = (MY == 16) ? 384 : 0, EX = (MY == 16) ? 640 : 2 * MX;
+        if (tid >= EY && tid < EY + 2 * MX) {
+            const int e = tid - EY;
+            const int a = e >> 1, b = (e & 1) ? MY : -1;
             double gx, gy, gz;
             flux(k, a, b, gx, gy, gz);
             sFy[bf][(b + 1) * MX + a] = gy;
-        } else if (tid < 2 * MX + 2 * MY) {
-            const int e = tid - 2 * MX;
+        } else if (tid >= EX && tid < EX + 2 * MY) {
+            const int e = tid - EX;
             const int b = e >> 1, a = (e & 1) ? MX : -1;
             double gx, gy, gz;
             flux(k, a, b, gx, gy, gz);
