@@ -196,8 +196,9 @@ typedef struct mhit_tuning {
     int32_t ac_mz;        /* z planes per workgroup, fused ACDI pass        */
     int32_t xcd;          /* 1: XCD-aware workgroup order, fused passes     */
     int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
-    int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8        */
-    int32_t ac_my;        /* tile rows of the fused ACDI pass: 16 or 8      */
+    int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8 (64
+                             columns) or 32 (32 columns)                    */
+    int32_t ac_my;        /* tile rows of the fused ACDI pass: the same     */
     int32_t fft_wave;     /* 1: column transforms of eight columns (the z
                              pass; the y pass in FP64) in the wavefront form,
                              lane/register transposes by permlane and DPP

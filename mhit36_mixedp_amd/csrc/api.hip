@@ -87,8 +87,8 @@ static void parse_tuning(const char* s, Tuning& t) {
             const int val = std::atoi(kv.c_str() + eq + 1);
             if (key == "ns_mz" && val > 0) t.ns_mz = val;
             else if (key == "ac_mz" && val > 0) t.ac_mz = val;
-            else if (key == "ns_my" && (val == 8 || val == 16)) t.ns_my = val;
-            else if (key == "ac_my" && (val == 8 || val == 16)) t.ac_my = val;
+            else if (key == "ns_my" && (val == 8 || val == 16 || val == 32)) t.ns_my = val;
+            else if (key == "ac_my" && (val == 8 || val == 16 || val == 32)) t.ac_my = val;
             else if (key == "xcd") t.xcd = val != 0;
             else if (key == "sort" && val >= 0) t.sort_every = val;
             else if (key == "fft") t.fft_wave = val != 0;
@@ -113,7 +113,9 @@ static void parse_tuning(const char* s, Tuning& t) {
 // shortens until its launches fill the GPU.
 void default_tuning(mhit_ctx* c, bool batched, bool env) {
     const int n = c->n;
-    const int rows = (!batched && n <= 64) ? 8 : 16;
+    // 32 x 32 tiles where 64-column tiles would leave part of each row of
+    // tiles empty (96^3: a third of the threads)
+    const int rows = (n > 64 && n % 64 != 0 && n % 32 == 0) ? 32 : ((!batched && n <= 64) ? 8 : 16);
     int zc = std::min(128, n);
     if (!batched) {
         int cus = 0;
@@ -121,7 +123,7 @@ void default_tuning(mhit_ctx* c, bool batched, bool env) {
             cus <= 0)
             cus = 256;
         zc = std::min(128, std::max(4, n / 4));
-        const long long tiles = (long long)div_up(n, 64) * div_up(n, rows);
+        const long long tiles = (long long)div_up(n, tile_cols(rows)) * div_up(n, rows);
         while (tiles * div_up(n, zc) < cus && zc / 2 >= 4) zc /= 2;
     }
     c->tune.ns_mz = zc;

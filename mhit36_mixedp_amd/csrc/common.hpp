@@ -50,8 +50,10 @@ struct Tuning {
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
     int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
-    int ns_my = 16;       // tile rows (16 or 8), NS pass (mhit_create: 8 for n <= 64)
-    int ac_my = 16;       // tile rows (16 or 8), ACDI pass (mhit_create: 8 for n <= 64)
+    int ns_my = 16;       // tile rows, NS pass: 16 or 8 (64 columns) or 32 (32
+                          // columns); mhit_create: 8 for n <= 64, 32 where n
+                          // is not a multiple of 64 (96)
+    int ac_my = 16;       // tile rows, ACDI pass (the same choices)
     int fft_wave = 1;     // column transforms of eight columns in the wavefront
                           // form (lane/register transposes by permlane and DPP)
                           // where n = 64, 128, 256, 512; 0: register-direct form
@@ -71,6 +73,9 @@ struct Grid {
 };
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// Columns of a fused pass's tile for its Tuning rows (64 x 16, 64 x 8, 32 x 32).
+inline int tile_cols(int rows) { return rows == 32 ? 32 : 64; }
 
 // periodic neighbours
 __device__ __forceinline__ int wrap_p(int i, int n) { return i + 1 == n ? 0 : i + 1; }

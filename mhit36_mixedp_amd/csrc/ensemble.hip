@@ -187,7 +187,8 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
     return launch_finalize_batch(boxes.data(), (int)boxes.size(), ar, st);
 }
 
-// Launch shapes for boxes batched in an ensemble: 64 x 16 tiles and z
+// Launch shapes for boxes batched in an ensemble: 64 x 16 tiles (32 x 32
+// where n is not a multiple of 64) and z
 // chunks of min(n, 128) planes, then chunks halved size class by size
 // class (the class with the longest chunks first, ties to the larger grid;
 // never below 8 planes) until the NS launches of one step hold at least
@@ -218,7 +219,8 @@ static void batched_shapes(const std::vector<mhit_ctx*>& boxes) {
     auto workgroups = [&]() {
         long long t = 0;
         for (mhit_ctx* c : boxes)
-            t += (long long)div_up(c->n, 64) * div_up(c->n, c->tune.ns_my) * div_up(c->n, zc_of(c->n));
+            t += (long long)div_up(c->n, tile_cols(c->tune.ns_my)) * div_up(c->n, c->tune.ns_my) *
+                 div_up(c->n, zc_of(c->n));
         return t;
     };
     while (workgroups() < target) {
@@ -518,8 +520,8 @@ int mhit_ensemble_autotune(mhit_ensemble* e, int steps, double* ms_per_step) {
         };
         const std::vector<Param> params = {
             {&Tuning::ns_mz, &Tuning::ac_mz, zc},
-            {&Tuning::ns_my, nullptr, {16, 8}},
-            {&Tuning::ac_my, nullptr, {16, 8}},
+            {&Tuning::ns_my, nullptr, {16, 8, 32}},
+            {&Tuning::ac_my, nullptr, {16, 8, 32}},
             {&Tuning::fft_wave, nullptr, {1, 0}},
             {&Tuning::fft_ycols, nullptr, widths(0)},
             {&Tuning::fft_zcols, nullptr, widths(1)},

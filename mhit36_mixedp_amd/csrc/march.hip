@@ -47,10 +47,10 @@ __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 // two-point halo on the +x/+y sides for these evaluations, and the fused
 // pass reads its neighbours' previous rhs, so the new AB2 history goes to
 // the other buffer of a ping-pong pair.
-// Tile: 64 columns in x by TMY rows in y (Tuning::ns_my, 16 or 8), one
-// thread per column; z chunk: Tuning::ns_mz.  The velocity planes carry
-// x, y in [-1, M + 1], phi a two-point halo.
-constexpr int NS_MX = 64;
+// Tile: TMX columns in x by TMY rows in y (Tuning::ns_my: 64 x 16, 64 x 8,
+// or 32 x 32 for grids that are not a multiple of 64), one thread per
+// column; z chunk: Tuning::ns_mz.  The velocity planes carry x, y in
+// [-1, M + 1], phi a two-point halo.
 
 // Arguments of one box.  A launch covers one box (`one`) or, for an
 // ensemble, a table of boxes of the same size stacked along the grid's z.
@@ -66,8 +66,8 @@ struct NsArgs {
     int zc, xcd;
 };
 
-template <bool PHI, typename TD, int TMY>
-__global__ __launch_bounds__(NS_MX * TMY)
+template <bool PHI, typename TD, int TMY, int TMX>
+__global__ __launch_bounds__(TMX * TMY)
 void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const Blk blk = launch_blk(nbox);
     NsArgs<TD> A = one;
@@ -92,7 +92,7 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
     const Grid g = A.g;
     const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     const int zc = A.zc, xcd = A.xcd;
-    constexpr int MX = NS_MX, MY = TMY, W1 = MX + 3, H1 = MY + 3, W2 = MX + 4, H2 = MY + 4;
+    constexpr int MX = TMX, MY = TMY, W1 = MX + 3, H1 = MY + 3, W2 = MX + 4, H2 = MY + 4;
     constexpr int NT = MX * MY;
     constexpr int FXW = MX + 2;                          // fx at x = -1..MX
     __shared__ double su[3][H1 * W1], sv[3][H1 * W1], sw[3][H1 * W1];
@@ -339,9 +339,9 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
 // =====================================================================
 // ACDI
 // =====================================================================
-// Tile: 64 columns by TMY rows (Tuning::ac_my, 16 or 8); z chunk:
-// Tuning::ac_mz.  phi ring with a one-point halo, psi ring with two.
-constexpr int AC_MX = 64;
+// Tile: TMX columns by TMY rows (Tuning::ac_my: 64 x 16, 64 x 8 or
+// 32 x 32); z chunk: Tuning::ac_mz.  phi ring with a one-point halo, psi
+// ring with two.
 
 struct AcdiArgs {
     double* phi_out;
@@ -352,8 +352,8 @@ struct AcdiArgs {
     int zc, xcd;
 };
 
-template <int TMY>
-__global__ __launch_bounds__(AC_MX * TMY)
+template <int TMY, int TMX>
+__global__ __launch_bounds__(TMX * TMY)
 void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
     const Blk blk = launch_blk(nbox);
     AcdiArgs A = one;
@@ -367,7 +367,7 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
     const Grid g = A.g;
     const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     const int zc = A.zc, xcd = A.xcd;
-    constexpr int MX = AC_MX, MY = TMY, W1 = MX + 2, H1 = MY + 2, W2 = MX + 4, H2 = MY + 4;
+    constexpr int MX = TMX, MY = TMY, W1 = MX + 2, H1 = MY + 2, W2 = MX + 4, H2 = MY + 4;
     constexpr int NT = MX * MY;
     __shared__ double spsi[3][H2 * W2];
     __shared__ double sphi[4][H1 * W1];
@@ -465,7 +465,8 @@ void k_acdi_march(AcdiArgs one, const AcdiArgs* __restrict__ tab, int nbox) {
         // the extra work spreads over the SIMDs: y = -1 and y = MY rows on
         // threads [EY, EY + 2 MX), x = -1 and x = MX columns on
         // [EX, EX + 2 MY)
-        constexpr int EY = (MY == 16) ? 384 : 0, EX = (MY == 16) ? 640 : 2 * MX;
+        constexpr bool SPREAD = MX == 64 && MY == 16;
+        constexpr int EY = SPREAD ? 384 : 0, EX = SPREAD ? 640 : 2 * MX;
         if (tid >= EY && tid < EY + 2 * MX) {
             const int e = tid - EY;
             const int a = e >> 1, b = (e & 1) ? MY : -1;
@@ -596,13 +597,14 @@ void ns_swap(mhit_ctx* c) {
 template <bool PH, typename TD>
 static void go_ns(mhit_ctx* c, const NsArgs<TD>& one, const NsArgs<TD>* tab, int nb, hipStream_t st) {
     const int n = c->n;
-    auto go = [&](auto my) {
-        constexpr int MY = decltype(my)::value;
-        dim3 grid(div_up(n, NS_MX), div_up(n, MY), div_up(n, one.zc) * nb), block(NS_MX, MY, 1);
-        k_ns_march<PH, TD, MY><<<grid, block, 0, st>>>(one, tab, nb);
+    auto go = [&](auto my, auto mx) {
+        constexpr int MY = decltype(my)::value, MX = decltype(mx)::value;
+        dim3 grid(div_up(n, MX), div_up(n, MY), div_up(n, one.zc) * nb), block(MX, MY, 1);
+        k_ns_march<PH, TD, MY, MX><<<grid, block, 0, st>>>(one, tab, nb);
     };
-    if (c->tune.ns_my == 8) go(std::integral_constant<int, 8>{});
-    else go(std::integral_constant<int, 16>{});
+    if (c->tune.ns_my == 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 64>{});
+    else if (c->tune.ns_my == 32) go(std::integral_constant<int, 32>{}, std::integral_constant<int, 32>{});
+    else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 64>{});
 }
 
 // The divergence goes to the pressure buffer, which the x R2C pass reads
@@ -650,13 +652,14 @@ int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipSt
 
 static void go_acdi(mhit_ctx* c, const AcdiArgs& one, const AcdiArgs* tab, int nb, hipStream_t st) {
     const int n = c->n;
-    auto go = [&](auto my) {
-        constexpr int MY = decltype(my)::value;
-        dim3 grid(div_up(n, AC_MX), div_up(n, MY), div_up(n, one.zc) * nb), block(AC_MX, MY, 1);
-        k_acdi_march<MY><<<grid, block, 0, st>>>(one, tab, nb);
+    auto go = [&](auto my, auto mx) {
+        constexpr int MY = decltype(my)::value, MX = decltype(mx)::value;
+        dim3 grid(div_up(n, MX), div_up(n, MY), div_up(n, one.zc) * nb), block(MX, MY, 1);
+        k_acdi_march<MY, MX><<<grid, block, 0, st>>>(one, tab, nb);
     };
-    if (c->tune.ac_my == 8) go(std::integral_constant<int, 8>{});
-    else go(std::integral_constant<int, 16>{});
+    if (c->tune.ac_my == 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 64>{});
+    else if (c->tune.ac_my == 32) go(std::integral_constant<int, 32>{}, std::integral_constant<int, 32>{});
+    else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 64>{});
 }
 
 int launch_acdi_march(mhit_ctx* c) {

@@ -172,7 +172,15 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
     const int z = blk.y;
     const int tid = threadIdx.x;
     stage_twiddles<V, N, NT>(stw, tw, tid);
-
+    // row H: the kx = n/2 coefficients of each row pair (real parts)
+    if (tid < P) {
+        const V An = gload_once(nyq + (size_t)z * N + y0 + 2 * tid);
+        const V Bn = gload_once(nyq + (size_t)z * N + y0 + 2 * tid + 1);
+        V zh; zh.x = An.x; zh.y = Bn.x;
+        lds[lidx<LD, SW>(H, tid)] = zh;
+    }
+    // rows k and n - k; kx = 0 (real parts only) by selects, without a
+    // divergent branch: its two writes both go to row 0
 #pragma unroll
     for (int r = 0; r < (H * P + NT - 1) / NT; r++) {
         const int e = tid + r * NT;
@@ -181,19 +189,11 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
         const size_t ra = ((size_t)z * N + y0 + 2 * c) * H;
         const V A = gload_once(spec + ra + k);
         const V B = gload_once(spec + ra + H + k);
-        if (k == 0) {
-            V z0; z0.x = A.x; z0.y = B.x;
-            const V An = gload_once(nyq + (size_t)z * N + y0 + 2 * c);
-            const V Bn = gload_once(nyq + (size_t)z * N + y0 + 2 * c + 1);
-            V zh; zh.x = An.x; zh.y = Bn.x;
-            lds[lidx<LD, SW>(0, c)] = z0;
-            lds[lidx<LD, SW>(H, c)] = zh;
-        } else {
-            V zk; zk.x = A.x - B.y; zk.y = A.y + B.x;
-            V zm; zm.x = A.x + B.y; zm.y = B.x - A.y;
-            lds[lidx<LD, SW>(k, c)] = zk;
-            lds[lidx<LD, SW>(N - k, c)] = zm;
-        }
+        const bool k0 = k == 0;
+        V zk; zk.x = k0 ? A.x : A.x - B.y; zk.y = k0 ? B.x : A.y + B.x;
+        V zm; zm.x = k0 ? A.x : A.x + B.y; zm.y = k0 ? B.x : B.x - A.y;
+        lds[lidx<LD, SW>(k, c)] = zk;
+        lds[lidx<LD, SW>(k0 ? 0 : N - k, c)] = zm;
     }
     __syncthreads();
     fft_stages<V, N, 1, P, TPC, LD, true, N, SW>(lds, tid % P, tid / P, stw);

@@ -237,13 +237,17 @@ int launch_ns(mhit_ctx* c) {
 // ----------------------------------------------------------- correction
 // C, main.f90:942-962: u = u* - dt/rho (p - p(-1)) dxi with the pressure
 // promoted to FP64, plus per-block sums and maxima of the corrected u,v,w.
-// A workgroup owns a 64 x 8 tile of columns and marches a chunk of z
+// A workgroup owns a 64 x 8 (or 32 x 16) tile of columns and marches a chunk of z
 // planes (correct_zc): p(z-1) of its column stays in a register, the
 // plane's p with a one-point halo at -x and -y is staged in LDS, and the
 // next plane is fetched into registers during the update.  The number of partials is
 // fixed by the grid, so the reduction order is deterministic.
 constexpr int kCorrZ = 16;                    // z split of the diagnostics pass
-constexpr int CO_MX = 64, CO_MY = 8;
+constexpr int CO_NT = 512;                    // threads per workgroup
+// tile columns: 64, or 32 where n is not a multiple of 64 (96^3: a
+// 64-column tile would leave a third of the threads without a cell); rows
+// make up the workgroup
+static int co_mx(int n) { return (n > 64 && n % 64 != 0 && n % 32 == 0) ? 32 : 64; }
 // z planes per workgroup: the longest of 32, 16, ..., 2 that gives a box
 // at least 256 workgroups (256^3: 32, 128^3: 16, 96^3: 8, 64^3 and below:
 // 2; at 64^3 8-plane chunks left 192 CUs idle, 14.8 us), and 16 from
@@ -253,7 +257,7 @@ constexpr int CO_MX = 64, CO_MY = 8;
 // the reduction order, is fixed per grid size.
 static int correct_zc(int n) {
     if (n >= 512) return 16;
-    const long long tiles = (long long)div_up(n, CO_MX) * div_up(n, CO_MY);
+    const long long tiles = (long long)div_up(n, co_mx(n)) * div_up(n, CO_NT / co_mx(n));
     int zc = 32;
     while (zc > 2 && tiles * div_up(n, zc) < 256) zc /= 2;
     return zc;
@@ -261,7 +265,7 @@ static int correct_zc(int n) {
 
 static dim3 correct_grid(int n) {
     const int zc = correct_zc(n) < n ? correct_zc(n) : n;
-    return dim3(div_up(n, CO_MX), div_up(n, CO_MY), div_up(n, zc));
+    return dim3(div_up(n, co_mx(n)), div_up(n, CO_NT / co_mx(n)), div_up(n, zc));
 }
 
 // Arguments of one box (a batched launch reads a table of them).
@@ -275,8 +279,8 @@ struct CorrArgs {
     int zc, xcd;
 };
 
-template <typename T>
-__global__ __launch_bounds__(CO_MX * CO_MY)
+template <typename T, int CMX>
+__global__ __launch_bounds__(CO_NT)
 void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
     const Blk blk = launch_blk(nbox);
     CorrArgs<T> A = one;
@@ -292,7 +296,7 @@ void k_correct(CorrArgs<T> one, const CorrArgs<T>* __restrict__ tab, int nbox) {
     const Grid g = A.g;
     const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
     const int zc = A.zc, xcd = A.xcd;
-    constexpr int MX = CO_MX, MY = CO_MY, NT = MX * MY, W = MX + 1, H = MY + 1;
+    constexpr int MX = CMX, MY = CO_NT / CMX, NT = MX * MY, W = MX + 1, H = MY + 1;
     constexpr int NP = (W * H + NT - 1) / NT;
     // p of the plane, x = -1..MX-1, y = -1..MY-1; two buffers, so that the
     // next plane's commit needs no barrier after this plane's reads
@@ -393,9 +397,10 @@ static CorrArgs<T> corr_args(mhit_ctx* c) {
 
 template <typename T>
 static void go_correct(int n, const CorrArgs<T>& one, const CorrArgs<T>* tab, int nb, hipStream_t st) {
-    dim3 grid = correct_grid(n), block(CO_MX, CO_MY, 1);
+    dim3 grid = correct_grid(n);
     grid.z *= nb;
-    k_correct<T><<<grid, block, 0, st>>>(one, tab, nb);
+    if (co_mx(n) == 32) k_correct<T, 32><<<grid, dim3(32, CO_NT / 32, 1), 0, st>>>(one, tab, nb);
+    else k_correct<T, 64><<<grid, dim3(64, CO_NT / 64, 1), 0, st>>>(one, tab, nb);
 }
 
 int launch_correct(mhit_ctx* c) {

@@ -3,7 +3,7 @@
 // The reference tunes its decomposition at start-up with cuDecomp's
 // autotuner (multi/main.f90:80-136: process grid, transpose and halo
 // backends).  On one GPU the free choices are launch shapes instead: the
-// tile rows (64x16 or 64x8) and the z planes per workgroup of the two
+// tile shape (64x16, 64x8 or 32x32) and the z planes per workgroup of the two
 // fused marching passes, and the XCD-aware tile order.  None of them
 // changes a result bit (every cell is
 // computed by the same arithmetic in any shape; the correction's reduction
@@ -179,7 +179,7 @@ bool valid_width(const mhit_ctx* c, int pass, int w) {
 }
 
 bool valid(const mhit_ctx* c, const mhit_tuning* t) {
-    auto rows = [](int r) { return r == 8 || r == 16; };
+    auto rows = [](int r) { return r == 8 || r == 16 || r == 32; };
     return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
            rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1) &&
            valid_width(c, 0, t->fft_ycols) && valid_width(c, 1, t->fft_zcols) && valid_width(c, 2, t->fft_xrows);
@@ -255,9 +255,9 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
     };
     struct Param { int* field; std::vector<int> cand; bool active; };
     std::vector<Param> params = {
-        {&c->tune.ns_my, {16, 8}, true},
+        {&c->tune.ns_my, {16, 8, 32}, true},
         {&c->tune.ns_mz, zlist({4, 8, 16, 32, 64, 128}), true},
-        {&c->tune.ac_my, {16, 8}, c->P.phiflag != 0},
+        {&c->tune.ac_my, {16, 8, 32}, c->P.phiflag != 0},
         {&c->tune.ac_mz, zlist({4, 8, 16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
         {&c->tune.fft_wave, {1, 0}, true},

@@ -31,7 +31,7 @@ def test_autotune_restores_state_and_results_are_shape_independent():
                      fft_xrows=t["fft_xrows"])
         assert t["ms_per_step"] > 0
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
-        assert t["ns_my"] in (8, 16) and t["ac_my"] in (8, 16) and t["fft_wave"] in (0, 1)
+        assert t["ns_my"] in (8, 16, 32) and t["ac_my"] in (8, 16, 32) and t["fft_wave"] in (0, 1)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
         # every field, the last projection's pressure (4) included
         for f in range(1, 10):
@@ -58,6 +58,10 @@ def test_autotune_restores_state_and_results_are_shape_independent():
     # x passes: no wavefront form, so every row-pair width is bit-neutral
     (64, dict(fft_xrows=4)),
     (96, dict(fft_xrows=16)),
+    # 32 x 32 tiles (the default at 96^3), full and partial (48 = 32 + 16)
+    (64, dict(ns_my=32, ac_my=32, ns_mz=16, ac_mz=16)),
+    (96, dict(ns_my=16, ac_my=8)),
+    (48, dict(ns_my=32, ac_my=32, ns_mz=8, ac_mz=4, xcd=1)),
 ])
 def test_set_tuning_is_bitwise_neutral(n, shape):
     from mhit36_mixedp_amd import Solver
