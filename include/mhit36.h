@@ -203,9 +203,11 @@ typedef struct mhit_tuning {
                              pass; the y pass in FP64) in the wavefront form,
                              lane/register transposes by permlane and DPP
                              (n = 64, 128, 256, 512); 0: the register-direct
-                             form with LDS exchanges.  The two forms compute
-                             the same DFT in a different operation order, so
-                             they differ in rounding only.                  */
+                             form with LDS exchanges (the default with the
+                             FP64 Poisson, where it measured faster).  The
+                             two forms compute the same DFT in a different
+                             operation order, so they differ in rounding
+                             only.                                          */
     int32_t fft_ycols;    /* columns per workgroup of the y passes          */
     int32_t fft_zcols;    /* columns per workgroup of the z pass            */
     int32_t fft_xrows;    /* row pairs per workgroup of the x passes: the FFT
