@@ -222,6 +222,14 @@ typedef struct mhit_tuning {
                              width that moves a pass to or from 8 columns
                              changes rounding; the other fields change no
                              result bit.                                    */
+    int32_t fft_xy;       /* 1: the x and y transforms of a z plane in one
+                             launch (the plane's half spectrum staged in LDS;
+                             the spectrum crosses HBM once instead of twice)
+                             where compiled: n = 32, 48, 64, 96, 128 with the
+                             FP32 Poisson, n = 32, 48, 64, 96 with FP64 (the
+                             default there; mhit_get_tuning reports 0
+                             elsewhere).  The x launch widths then go unused.
+                             Changes rounding only.                         */
     double ms_per_step;   /* mhit_autotune: measured with these settings    */
 } mhit_tuning;
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);

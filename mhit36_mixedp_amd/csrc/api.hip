@@ -96,6 +96,7 @@ static void parse_tuning(const char* s, Tuning& t) {
             else if (key == "ycols" && val > 0) t.fft_ycols = val;
             else if (key == "zcols" && val > 0) t.fft_zcols = val;
             else if (key == "xrows" && val > 0) t.fft_xrows = val;
+            else if (key == "xy") t.fft_xy = val != 0;
         }
         pos = end + 1;
     }
@@ -134,6 +135,11 @@ void default_tuning(mhit_ctx* c, bool batched, bool env) {
     // with the FP64 Poisson the register-direct form measured faster
     // (512^3: 8.84-8.89 -> 8.77 ms per step)
     c->tune.fft_wave = c->fp32 ? 1 : 0;
+    // one launch for the x and y transforms where compiled: 64^3 FP64
+    // 0.051 -> 0.049 ms per step, 96^3 FP64 0.099 -> 0.095; batched 16 x
+    // 64^3 / 96^3 / 128^3 (FP32) 56.0k / 18.2k / 9.79k -> 58.2k / 18.7k /
+    // 9.91k box-steps/s
+    c->tune.fft_xy = fft_xy_supported(n, c->fp32) ? 1 : 0;
     if (env) tuning_env(c);
 }
 

@@ -62,6 +62,9 @@ struct Tuning {
     int fft_ycols = 0;    // columns per workgroup, y passes
     int fft_zcols = 0;    // columns per workgroup, z pass
     int fft_xrows = 0;    // row pairs per workgroup, x passes
+    int fft_xy = 0;       // x and y transforms of a z plane in one launch where
+                          // compiled (n = 32 ... 128 FP32, 32 ... 96 FP64);
+                          // mhit_create: 1 there
 };
 
 // Per-run constants passed by value to kernels.

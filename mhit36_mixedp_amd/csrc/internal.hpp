@@ -209,6 +209,9 @@ void dump_release(mhit_ctx* c);
 
 // Supported grid sizes (FFT lengths).
 bool fft_size_supported(int n);
+// poisson.hip: whether the x and y transforms of a z plane run as one launch
+// for this size and precision (Tuning::fft_xy)
+bool fft_xy_supported(int n, bool fp32);
 // Launch widths compiled for an FFT pass (0: y passes, columns per
 // workgroup; 1: z pass, columns; 2: x passes, row pairs) of size n and
 // precision: out[0] is the default, the count is returned (at most 3).

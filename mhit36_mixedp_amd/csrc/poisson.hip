@@ -208,6 +208,216 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
     }
 }
 
+// ------------------------------------------- x and y passes in one launch
+// For n <= 128 a z plane's half spectrum fits in LDS, so one workgroup per
+// plane runs the x transforms of all its rows and the y transforms of all
+// its columns, and the spectrum goes to HBM once instead of twice
+// (Tuning::fft_xy).  Forward: the N rows enter the x R2C as P = N/2 row
+// pairs (columns of the LDS array, as in k_fft_x_fwd); the split rewrites
+// the same array as the half spectrum sy[y][kx] (N/2 columns kx of length
+// N; every thread holds its split values in registers across a barrier),
+// and the y transforms run on those columns.  The kx = 0 and kx = n/2
+// coefficients of every row are real, so column kx = 0 carries both,
+// X[y][0] + i X[y][n/2]; after the y transform they are separated by
+// Hermitian symmetry as the x pass separates a row pair.  Inverse: the
+// same steps backwards (column 0 takes F[ky][0] + i F[ky][n/2], whose
+// inverse along y has the two real columns as its real and imaginary
+// parts).  One array of n (n/2 + 1) values; compiled where two workgroups
+// fit a CU's LDS (n <= 128 FP32, n <= 96 FP64: FP64 at 128^3 measured
+// slower with one, 0.190 -> 0.193 ms per multiphase step).
+template <typename T, int N>
+constexpr bool xy_fits() {
+    return N % 8 == 0 && N >= 32 && N <= 128 &&
+           (N * (N / 2 + 1) + N) * (int)sizeof(cplx_t<T>) <= 80 * 1024;
+}
+
+template <typename T, int N, int IN>
+__global__ __launch_bounds__((N / 2) * (N / 8))
+void k_fft_xy_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox) {
+    using V = cplx_t<T>;
+    constexpr int P = N / 2, H = N / 2, TPC = N / 8, NT = P * TPC;
+    constexpr bool POW2 = (P & (P - 1)) == 0;
+    constexpr int LD = POW2 ? P : P + 1, SW = POW2 ? P - 1 : 0;
+    constexpr int NSP = N * H / NT;   // split values per thread (8)
+    static_assert(N * H % NT == 0 && H <= LD, "one plane per workgroup");
+    __shared__ V lds[N * LD];   // x stage: row pair c as column c; then sy[y][kx]
+    __shared__ V stw[N];
+    V* const sy = lds;
+    const Blk blk = launch_blk(nbox);
+    XFwdArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL double* __restrict__ us = gmem(A.us);
+    const MHIT_GLOBAL double* __restrict__ vs = gmem(A.vs);
+    const MHIT_GLOBAL double* __restrict__ ws = gmem(A.ws);
+    const MHIT_GLOBAL T* __restrict__ din = gmem(A.din);
+    const double cdiv = A.cdiv;
+    MHIT_GLOBAL V* __restrict__ spec = gmem(A.spec);
+    MHIT_GLOBAL V* __restrict__ nyq = gmem(A.nyq);
+    const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    if (S && S->halted) return;
+    const int z = blk.y;
+    const int tid = threadIdx.x;
+    const int zp = (z + 1 == N) ? 0 : z + 1;
+    stage_twiddles<V, N, NT>(stw, tw, tid);
+#pragma unroll
+    for (int r = 0; r < N * P / NT; r++) {
+        const int e = tid + r * NT;
+        const int a = e % N, c = e / N;
+        V val;
+        if (IN == X_IN_READY) {
+            val.x = ld_once(&din[idx3(a, 2 * c, z, N)]);
+            val.y = ld_once(&din[idx3(a, 2 * c + 1, z, N)]);
+        } else {
+            double d[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int y = 2 * c + h;
+                const size_t q = idx3(a, y, z, N);
+                if (IN == X_IN_DIV) {
+                    // rhsp, main.f90:841-843
+                    const int ap = (a + 1 == N) ? 0 : a + 1;
+                    const int yp = (y + 1 == N) ? 0 : y + 1;
+                    double rr = cdiv * (us[idx3(ap, y, z, N)] - us[q]);
+                    rr = rr + cdiv * (vs[idx3(a, yp, z, N)] - vs[q]);
+                    rr = rr + cdiv * (ws[idx3(a, y, zp, N)] - ws[q]);
+                    d[h] = rr;
+                } else {
+                    d[h] = us[q];
+                }
+            }
+            val.x = (T)d[0];   // rhspp = real(rhsp,4) in the FP32 form (main.f90:844)
+            val.y = (T)d[1];
+        }
+        lds[lidx<LD, SW>(a, c)] = val;
+    }
+    __syncthreads();
+    fft_stages<V, N, 1, P, TPC, LD, false, N, SW>(lds, tid % P, tid / P, stw);
+    // split into rows y: X_y[k], k < H; column 0 takes X_y[0] + i X_y[H]
+    V sp[NSP];
+#pragma unroll
+    for (int r = 0; r < NSP; r++) {
+        const int e = tid + r * NT;
+        const int k = e % H, y = e / H, cc = y >> 1;
+        const V zk = lds[lidx<LD, SW>(k, cc)];
+        const V zm = lds[lidx<LD, SW>((N - k) % N, cc)];
+        const V zh = lds[lidx<LD, SW>(H, cc)];
+        V out;
+        if ((y & 1) == 0) {
+            out.x = (zk.x + zm.x) * (T)0.5;
+            out.y = (zk.y - zm.y) * (T)0.5;
+        } else {
+            out.x = (zk.y + zm.y) * (T)0.5;
+            out.y = (zm.x - zk.x) * (T)0.5;
+        }
+        if (k == 0) out.y = (y & 1) ? zh.y : zh.x;   // X_y[0] is real: its imaginary slot takes X_y[H]
+        sp[r] = out;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < NSP; r++) sy[tid + r * NT] = sp[r];   // sy[y][k] at y H + k = e
+    __syncthreads();
+    fft_stages<V, N, 1, H, TPC, H, false>(sy, tid % H, tid / H, stw);
+    // store spec[z][ky][kx]; kx = 0 and the Nyquist plane separated from column 0
+#pragma unroll
+    for (int r = 0; r < NSP; r++) {
+        const int e = tid + r * NT;
+        const int kx = e % H, ky = e / H;
+        V out = sy[e];
+        if (kx == 0) {
+            const V cm = sy[((N - ky) % N) * H];
+            const V c0 = out;
+            out.x = (c0.x + cm.x) * (T)0.5;
+            out.y = (c0.y - cm.y) * (T)0.5;
+            V fn;
+            fn.x = (c0.y + cm.y) * (T)0.5;
+            fn.y = (cm.x - c0.x) * (T)0.5;
+            gstore_once(nyq + (size_t)z * N + ky, fn);
+        }
+        gstore_once(spec + ((size_t)z * N + ky) * H + kx, out);
+    }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__((N / 2) * (N / 8))
+void k_fft_yx_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox) {
+    using V = cplx_t<T>;
+    constexpr int P = N / 2, H = N / 2, TPC = N / 8, NT = P * TPC;
+    constexpr bool POW2 = (P & (P - 1)) == 0;
+    constexpr int LD = POW2 ? P : P + 1, SW = POW2 ? P - 1 : 0;
+    constexpr int NSP = N * H / NT;   // spectrum values per thread (8)
+    constexpr int NPR = H * P / NT;   // row-pair coefficients per thread (4)
+    static_assert(N * H % NT == 0 && H * P % NT == 0 && P <= NT && H <= LD, "one plane per workgroup");
+    __shared__ V lds[N * LD];   // sy[ky][kx]; then the x stage, row pair c as column c
+    __shared__ V stw[N];
+    V* const sy = lds;
+    const Blk blk = launch_blk(nbox);
+    XInvArgs<T> A = one;
+    if (tab) A = tab[blk.box];
+    const MHIT_GLOBAL V* __restrict__ spec = gmem(A.spec);
+    const MHIT_GLOBAL V* __restrict__ nyq = gmem(A.nyq);
+    MHIT_GLOBAL T* __restrict__ pres = gmem(A.pres);
+    const MHIT_GLOBAL V* __restrict__ tw = gmem(A.tw);
+    const MHIT_GLOBAL DevScalars* __restrict__ S = gmem(A.S);
+    if (S && S->halted) return;
+    const int z = blk.y;
+    const int tid = threadIdx.x;
+    stage_twiddles<V, N, NT>(stw, tw, tid);
+    // the plane's spectrum into sy[ky][kx]; column 0 takes F[ky][0] + i F[ky][H]
+#pragma unroll
+    for (int r = 0; r < NSP; r++) {
+        const int e = tid + r * NT;
+        const int kx = e % H, ky = e / H;
+        V v = gload_once(spec + (size_t)z * N * H + e);
+        const V fn = gload(nyq + (size_t)z * N + ky);   // one line per row, shared by its lanes
+        if (kx == 0) {
+            const V f0 = v;
+            v.x = f0.x - fn.y;
+            v.y = f0.y + fn.x;
+        }
+        sy[e] = v;
+    }
+    __syncthreads();
+    fft_stages<V, N, 1, H, TPC, H, true>(sy, tid % H, tid / H, stw);
+    // x C2R input of row pair c (as k_fft_x_inv): Z[k] = G[2c][k] + i G[2c+1][k]
+    // over the Hermitian extension; G[y][0] = Re sy[y][0], G[y][H] = Im sy[y][0]
+    V zh;
+    if (tid < P) {
+        const V a0 = sy[(2 * tid) * H], b0 = sy[(2 * tid + 1) * H];
+        zh.x = a0.y;
+        zh.y = b0.y;
+    }
+    V zk[NPR], zm[NPR];
+#pragma unroll
+    for (int r = 0; r < NPR; r++) {
+        const int e = tid + r * NT;
+        const int k = e % H, c = e / H;
+        const V Av = sy[(2 * c) * H + k];
+        const V Bv = sy[(2 * c + 1) * H + k];
+        const bool k0 = k == 0;
+        zk[r].x = k0 ? Av.x : Av.x - Bv.y; zk[r].y = k0 ? Bv.x : Av.y + Bv.x;
+        zm[r].x = k0 ? Av.x : Av.x + Bv.y; zm[r].y = k0 ? Bv.x : Bv.x - Av.y;
+    }
+    __syncthreads();
+    if (tid < P) lds[lidx<LD, SW>(H, tid)] = zh;
+#pragma unroll
+    for (int r = 0; r < NPR; r++) {
+        const int e = tid + r * NT;
+        const int k = e % H, c = e / H;
+        lds[lidx<LD, SW>(k, c)] = zk[r];
+        lds[lidx<LD, SW>(k == 0 ? 0 : N - k, c)] = zm[r];
+    }
+    __syncthreads();
+    fft_stages<V, N, 1, P, TPC, LD, true, N, SW>(lds, tid % P, tid / P, stw);
+#pragma unroll
+    for (int r = 0; r < N * 2 * P / NT; r++) {
+        const int e = tid + r * NT;
+        const int x = e % N, rr = e / N;
+        const V zz = lds[lidx<LD, SW>(x, rr >> 1)];
+        st_once(&pres[idx3(x, rr, z, N)], (rr & 1) ? zz.y : zz.x);
+    }
+}
+
 // ------------------------------------------------------------ column pass
 // C columns of length N, element (c, a) at data[base + a*sa + c*sc] with
 // base = bxi*bx + byi*by.  MODE 0 forward, 1 inverse, 2 forward + spectral
@@ -717,43 +927,111 @@ struct PoissonPasses {
         k_fft_x_inv<T, N, P><<<grid, block, 0, st>>>(one, tab, nb);
         return MHIT_OK;
     }
+    // x and y passes in one launch per z plane (Tuning::fft_xy) where the
+    // plane fits in LDS
+    static constexpr bool XY = xy_fits<T, N>();
+    static bool use_xy(const mhit_ctx* c) { return XY && c->tune.fft_xy; }
+    static int xy_fwd(Boxes bx, int nb, ArgArena* ar, hipStream_t st, const double* a, const double* b,
+                      const double* w, int in) {
+        if constexpr (XY) {
+            XFwdArgs<T> one;
+            const XFwdArgs<T>* tab;
+            int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+                XFwdArgs<T> x;
+                x.us = in == X_IN_READY ? nullptr : a;
+                x.vs = in == X_IN_DIV ? b : nullptr;
+                x.ws = in == X_IN_DIV ? w : nullptr;
+                x.din = in == X_IN_READY ? (const T*)c->pres : nullptr;
+                x.cdiv = in == X_IN_DIV ? c->g.rho * c->g.dxi / c->g.dt : 0.0;
+                x.spec = (V*)c->spec;
+                x.nyq = (V*)c->nyq;
+                x.tw = (const V*)c->tw;
+                x.S = c->S;
+                return x;
+            }, one, tab);
+            if (rc) return rc;
+            dim3 grid(1, N, nb), block((N / 2) * (N / 8));
+            if (in == X_IN_READY) k_fft_xy_fwd<T, N, X_IN_READY><<<grid, block, 0, st>>>(one, tab, nb);
+            else if (in == X_IN_DIV) k_fft_xy_fwd<T, N, X_IN_DIV><<<grid, block, 0, st>>>(one, tab, nb);
+            else k_fft_xy_fwd<T, N, X_IN_FIELD><<<grid, block, 0, st>>>(one, tab, nb);
+            return MHIT_OK;
+        } else {
+            set_error("fused x/y transform not compiled for this size");
+            return MHIT_ERR_STATE;
+        }
+    }
+    static int yx_inv(Boxes bx, int nb, ArgArena* ar, hipStream_t st) {
+        if constexpr (XY) {
+            XInvArgs<T> one;
+            const XInvArgs<T>* tab;
+            int rc = with_table(bx, nb, ar, [&](mhit_ctx* c) {
+                return XInvArgs<T>{(const V*)c->spec, (const V*)c->nyq, (T*)c->pres, (const V*)c->tw, c->S};
+            }, one, tab);
+            if (rc) return rc;
+            dim3 grid(1, N, nb), block((N / 2) * (N / 8));
+            k_fft_yx_inv<T, N><<<grid, block, 0, st>>>(one, tab, nb);
+            return MHIT_OK;
+        } else {
+            set_error("fused x/y transform not compiled for this size");
+            return MHIT_ERR_STATE;
+        }
+    }
+    // x then y forward, or y then x inverse, fused or as two passes
+    // (mark: stage events for the per-stage profile of a single box; the
+    // fused launch is timed as the x stage, its y stage as zero)
+    static int fwd_xy(Boxes bx, int nb, ArgArena* ar, hipStream_t st, const double* a, const double* b,
+                      const double* w, int in, bool mark) {
+        if (use_xy(bx[0])) {
+            const int rc = xy_fwd(bx, nb, ar, st, a, b, w, in);
+            if (mark) { prof_mark(bx[0], ST_FFT_X_FWD); prof_mark(bx[0], ST_FFT_Y_FWD); }
+            return rc;
+        }
+        int rc;
+        if ((rc = x_fwd(bx, nb, ar, st, a, b, w, in))) return rc;
+        if (mark) prof_mark(bx[0], ST_FFT_X_FWD);
+        if ((rc = y_pass<0>(bx, nb, ar, st))) return rc;
+        if (mark) prof_mark(bx[0], ST_FFT_Y_FWD);
+        return MHIT_OK;
+    }
+    static int inv_yx(Boxes bx, int nb, ArgArena* ar, hipStream_t st, bool mark) {
+        if (use_xy(bx[0])) {
+            const int rc = yx_inv(bx, nb, ar, st);
+            if (mark) { prof_mark(bx[0], ST_FFT_Y_INV); prof_mark(bx[0], ST_FFT_X_INV); }
+            return rc;
+        }
+        int rc;
+        if ((rc = y_pass<1>(bx, nb, ar, st))) return rc;
+        if (mark) prof_mark(bx[0], ST_FFT_Y_INV);
+        if ((rc = x_inv(bx, nb, ar, st))) return rc;
+        if (mark) prof_mark(bx[0], ST_FFT_X_INV);
+        return MHIT_OK;
+    }
     static int project(mhit_ctx* c, const double* a, const double* b, const double* w, int in) {
         mhit_ctx* const bx[1] = {c};
         hipStream_t st = c->stream;
         int rc;
-        if ((rc = x_fwd(bx, 1, nullptr, st, a, b, w, in))) return rc;
-        prof_mark(c, ST_FFT_X_FWD);
-        if ((rc = y_pass<0>(bx, 1, nullptr, st))) return rc;
-        prof_mark(c, ST_FFT_Y_FWD);
+        if ((rc = fwd_xy(bx, 1, nullptr, st, a, b, w, in, true))) return rc;
         if ((rc = z_pass<2>(bx, 1, nullptr, st))) return rc;
         prof_mark(c, ST_FFT_Z);
-        if ((rc = y_pass<1>(bx, 1, nullptr, st))) return rc;
-        prof_mark(c, ST_FFT_Y_INV);
-        if ((rc = x_inv(bx, 1, nullptr, st))) return rc;
-        prof_mark(c, ST_FFT_X_INV);
-        return MHIT_OK;
+        return inv_yx(bx, 1, nullptr, st, true);
     }
     static int project_batch(Boxes bx, int nb, ArgArena& ar, hipStream_t st) {
         int rc;
-        if ((rc = x_fwd(bx, nb, &ar, st, nullptr, nullptr, nullptr, X_IN_READY))) return rc;
-        if ((rc = y_pass<0>(bx, nb, &ar, st))) return rc;
+        if ((rc = fwd_xy(bx, nb, &ar, st, nullptr, nullptr, nullptr, X_IN_READY, false))) return rc;
         if ((rc = z_pass<2>(bx, nb, &ar, st))) return rc;
-        if ((rc = y_pass<1>(bx, nb, &ar, st))) return rc;
-        return x_inv(bx, nb, &ar, st);
+        return inv_yx(bx, nb, &ar, st, false);
     }
     static int r2c(mhit_ctx* c, const double* in) {
         mhit_ctx* const bx[1] = {c};
         int rc;
-        if ((rc = x_fwd(bx, 1, nullptr, c->stream, in, nullptr, nullptr, X_IN_FIELD))) return rc;
-        if ((rc = y_pass<0>(bx, 1, nullptr, c->stream))) return rc;
+        if ((rc = fwd_xy(bx, 1, nullptr, c->stream, in, nullptr, nullptr, X_IN_FIELD, false))) return rc;
         return z_pass<0>(bx, 1, nullptr, c->stream);
     }
     static int c2r(mhit_ctx* c) {
         mhit_ctx* const bx[1] = {c};
         int rc;
         if ((rc = z_pass<1>(bx, 1, nullptr, c->stream))) return rc;
-        if ((rc = y_pass<1>(bx, 1, nullptr, c->stream))) return rc;
-        return x_inv(bx, 1, nullptr, c->stream);
+        return inv_yx(bx, 1, nullptr, c->stream, false);
     }
 };
 
@@ -765,6 +1043,16 @@ bool fft_size_supported(int n) {
         MHIT_FOR_EACH_N(CASE)
 #undef CASE
         return true;
+        default: return false;
+    }
+}
+
+bool fft_xy_supported(int n, bool fp32) {
+    switch (n) {
+#define CASE(NN) \
+    case NN: return fp32 ? xy_fits<float, NN>() : xy_fits<double, NN>();
+        MHIT_FOR_EACH_N(CASE)
+#undef CASE
         default: return false;
     }
 }

@@ -154,7 +154,8 @@ def test_every_launch_width_matches_numpy(n, prec, form):
     """The alternative FFT launch widths (Tuning fft_ycols / fft_zcols /
     fft_xrows, one at a time) against numpy: forward, inverse and the
     Poisson solve, in both transform forms (eight columns take the
-    wavefront form when it is on)."""
+    wavefront form when it is on), x and y transforms in separate
+    launches."""
     print(f"[fft widths] n={n} prec={prec} form={form}", flush=True)
     rng = np.random.default_rng(700 + n + form)
     x = rng.standard_normal((n, n, n))
@@ -174,10 +175,49 @@ def test_every_launch_width_matches_numpy(n, prec, form):
         if (n // 2) % (2 * base["fft_xrows"]) == 0:
             alts.append(dict(fft_xrows=base["fft_xrows"] * 2))
         for alt in alts:
-            kw = dict(fft_wave=form, fft_ycols=0, fft_zcols=0, fft_xrows=0)
+            kw = dict(fft_wave=form, fft_ycols=0, fft_zcols=0, fft_xrows=0, fft_xy=0)
             kw.update(alt)
             s.set_tuning(**kw)
             assert all(s.tuning()[kk] == v for kk, v in alt.items())
             assert relerr(s.fft_r2c(x), np.fft.rfftn(x)) < tol, alt
             assert relerr(s.fft_c2r(np.fft.rfftn(x)), x * n ** 3) < tol, alt
             assert relerr(s.poisson_solve(x), ref) < (1e-12 if prec == 0 else 1e-5), alt
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [32, 48, 64, 96, 128])
+def test_xy_in_one_launch_and_apart_match_numpy(n, prec):
+    """The x and y transforms of a plane in one launch (fft_xy = 1, the
+    default where compiled) and in two: forward, inverse and the Poisson
+    solve against numpy, and the two forms against each other at the
+    transform tolerance.  The one-launch form packs the kx = 0 and
+    kx = n/2 columns into one complex column and separates them after the
+    y transform, so a wrong pack or split fails the Nyquist plane of the
+    forward transform and the real rows of the inverse."""
+    print(f"[fft xy] n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(900 + n + prec)
+    x = rng.standard_normal((n, n, n))
+    tol = 1e-12 if prec == 0 else 2e-6
+    X = np.fft.rfftn(x)
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kz, ky, kx = np.meshgrid(k, k, k[: n // 2 + 1], indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k2[0, 0, 0] = 1.0
+    ph = -X / k2
+    ph[0, 0, 0] = 0.0
+    ref = np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2))
+    out = {}
+    with _solver(n, prec) as s:
+        for v in (1, 0):
+            s.set_tuning(fft_xy=v)
+            assert s.tuning()["fft_xy"] == (v if (prec == 1 or n <= 96) else 0)
+            F = s.fft_r2c(x)
+            assert relerr(F, X) < tol, v
+            assert relerr(F[:, :, n // 2], X[:, :, n // 2]) < tol, v   # the Nyquist plane
+            y = s.fft_c2r(X)
+            assert relerr(y, x * n ** 3) < tol, v
+            p = s.poisson_solve(x)
+            assert relerr(p, ref) < (1e-12 if prec == 0 else 1e-5), v
+            out[v] = (F, y, p)
+    for a, b in zip(out[1], out[0]):
+        assert relerr(a, b) < tol

@@ -1,6 +1,6 @@
 """The autotuner leaves the state as it found it, and no launch shape
-changes a result bit (the transform form, fft_wave, changes rounding
-only; tests/test_gpu_fft.py checks both forms)."""
+changes a result bit (the transform forms, fft_wave and fft_xy, change
+rounding only; tests/test_gpu_fft.py checks every form)."""
 import numpy as np
 import pytest
 
@@ -28,8 +28,8 @@ def test_autotune_restores_state_and_results_are_shape_independent():
         # are the FFT widths, compiled with alternatives from 64 up); the
         # launch shapes change no bit
         b.set_tuning(fft_wave=t["fft_wave"], fft_ycols=t["fft_ycols"], fft_zcols=t["fft_zcols"],
-                     fft_xrows=t["fft_xrows"])
-        assert t["ms_per_step"] > 0
+                     fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"])
+        assert t["ms_per_step"] > 0 and t["fft_xy"] in (0, 1)
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
         assert t["ns_my"] in (8, 16, 32) and t["ac_my"] in (8, 16, 32) and t["fft_wave"] in (0, 1)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
@@ -56,8 +56,9 @@ def test_autotune_restores_state_and_results_are_shape_independent():
     # 24 rows: the 16-row tiles end in a partial tile (the default is 8)
     (24, dict(ns_mz=8, ac_mz=8, xcd=0, ns_my=16, ac_my=16)),
     # x passes: no wavefront form, so every row-pair width is bit-neutral
-    (64, dict(fft_xrows=4)),
-    (96, dict(fft_xrows=16)),
+    # (with the x and y transforms in separate launches, on both solvers)
+    (64, dict(fft_xrows=4, fft_xy=0)),
+    (96, dict(fft_xrows=16, fft_xy=0)),
     # 32 x 32 tiles (the default at 96^3), full and partial (48 = 32 + 16)
     (64, dict(ns_my=32, ac_my=32, ns_mz=16, ac_mz=16)),
     (96, dict(ns_my=16, ac_my=8)),
@@ -69,6 +70,8 @@ def test_set_tuning_is_bitwise_neutral(n, shape):
     kw = dict(nx=n, phiflag=1, sigma=0.5, dt=5e-4, poisson_precision=1)
     with Solver(**kw) as a, Solver(**kw) as b:
         a.set_tuning(**shape)
+        # the transform forms change rounding: the same on both
+        b.set_tuning(**{k: v for k, v in shape.items() if k in ("fft_wave", "fft_xy")})
         for s in (a, b):
             s.init_random_velocity(4, kmax=4.0, urms=1.0)
             s.init_drops(2, 0.7, 4)
@@ -85,11 +88,11 @@ def test_mhit_tune_overrides_and_ignores_bad_values(monkeypatch):
     print("[tuning] MHIT_TUNE parsing", flush=True)
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         base = s.tuning()
-    monkeypatch.setenv("MHIT_TUNE", "ns_mz=16,ac_mz=8,ns_my=16,ac_my=16,xcd=1,sort=3,fft=0")
+    monkeypatch.setenv("MHIT_TUNE", "ns_mz=16,ac_mz=8,ns_my=16,ac_my=16,xcd=1,sort=3,fft=0,xy=0")
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         t = s.tuning()
-    assert (t["ns_mz"], t["ac_mz"], t["ns_my"], t["ac_my"], t["xcd"], t["sort_every"], t["fft_wave"]) == \
-        (16, 8, 16, 16, 1, 3, 0)
+    assert (t["ns_mz"], t["ac_mz"], t["ns_my"], t["ac_my"], t["xcd"], t["sort_every"], t["fft_wave"],
+            t["fft_xy"]) == (16, 8, 16, 16, 1, 3, 0, 0)
     monkeypatch.setenv("MHIT_TUNE", "ns_mz=0,ac_mz=-4,ns_my=12,ac_my=4,bogus=7,sort=-1")
     with Solver(nx=32, phiflag=1, sigma=0.5) as s:
         t = s.tuning()
@@ -111,8 +114,8 @@ def test_fft_widths_are_bitwise_neutral_in_one_form(n, prec):
         alt = dict(fft_ycols=base["fft_ycols"] // 2,
                    fft_zcols=base["fft_zcols"] * 2 if prec == 1 else base["fft_zcols"] // 2,
                    fft_xrows=base["fft_xrows"] // 2)
-        a.set_tuning(fft_wave=0, **alt)
-        b.set_tuning(fft_wave=0)
+        a.set_tuning(fft_wave=0, fft_xy=0, **alt)
+        b.set_tuning(fft_wave=0, fft_xy=0)
         t = a.tuning()
         assert all(t[k] == v for k, v in alt.items()), (t, alt)
         for s in (a, b):
@@ -140,3 +143,22 @@ def test_fft_width_validation():
     with Solver(nx=32, poisson_precision=1) as s:   # no alternatives below 64
         with pytest.raises(MhitError):
             s.set_tuning(fft_xrows=4)
+
+
+@pytest.mark.parametrize("n,prec,compiled", [(24, 1, False), (32, 1, True), (64, 0, True), (96, 0, True),
+                                             (128, 1, True), (128, 0, False), (256, 1, False)])
+def test_fft_xy_default_and_validation(n, prec, compiled):
+    """The one-launch x/y transforms are the default exactly where they are
+    compiled (n = 32 ... 128 FP32, 32 ... 96 FP64); elsewhere the
+    context reports 0 whatever was asked; values other than 0 and 1 are
+    rejected."""
+    from mhit36_mixedp_amd import MhitError, Solver
+    print(f"[tuning] fft_xy n={n} prec={prec}", flush=True)
+    with Solver(nx=n, poisson_precision=prec) as s:
+        assert s.tuning()["fft_xy"] == int(compiled)
+        s.set_tuning(fft_xy=0)
+        assert s.tuning()["fft_xy"] == 0
+        s.set_tuning(fft_xy=1)
+        assert s.tuning()["fft_xy"] == int(compiled)
+        with pytest.raises(MhitError):
+            s.set_tuning(fft_xy=2)
