@@ -20,6 +20,8 @@
 //                 on spec and on nyq (one launch)
 //   Y2:           inverse C2C along y (one launch)
 //   X2 (x_inv):   C2R along x, writes pres
+// For n <= 128 (FP32; 96 in FP64) X1 + Y1 and Y2 + X2 are one launch each,
+// one workgroup per z plane (k_fft_xy_fwd, k_fft_yx_inv).
 #include "fft.hpp"
 #include "internal.hpp"
 
