@@ -230,6 +230,13 @@ typedef struct mhit_tuning {
                              default there; mhit_get_tuning reports 0
                              elsewhere).  The x launch widths then go unused.
                              Changes rounding only.                         */
+    int32_t fft_xplan;    /* radix plan of the separate x passes: 0 radix 8
+                             first (512 = 8.8.8, 256 = 8.4.8, 128 = 8.2.8,
+                             the default), 1 radix 4 until 64 points remain
+                             (512 = 4.4.4.8, 256 = 4.8.8, 128 = 4.4.8,
+                             384 = 3.4.4.8).  Only n = 128, 256, 384, 512
+                             have two plans (mhit_get_tuning reports 0
+                             elsewhere).  Changes rounding only.            */
     double ms_per_step;   /* mhit_autotune: measured with these settings    */
 } mhit_tuning;
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);

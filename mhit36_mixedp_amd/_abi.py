@@ -95,8 +95,9 @@ class Tuning(ctypes.Structure):
     """mhit_tuning: launch shapes (no result bit changes), the column
     transforms' form (fft_wave: rounding-level differences only) and the FFT
     launch widths (0 = the default; a width to or from eight columns moves
-    a pass to or from the wavefront form), and whether the x and y
-    transforms of a plane share one launch (fft_xy: rounding only)."""
+    a pass to or from the wavefront form), whether the x and y transforms
+    of a plane share one launch (fft_xy) and the x passes' radix plan
+    (fft_xplan); those two change rounding only."""
 
     _fields_ = [
         ("ns_mz", ctypes.c_int32),
@@ -110,6 +111,7 @@ class Tuning(ctypes.Structure):
         ("fft_zcols", ctypes.c_int32),
         ("fft_xrows", ctypes.c_int32),
         ("fft_xy", ctypes.c_int32),
+        ("fft_xplan", ctypes.c_int32),
         ("ms_per_step", ctypes.c_double),
     ]
 

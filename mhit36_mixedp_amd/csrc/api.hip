@@ -97,6 +97,7 @@ static void parse_tuning(const char* s, Tuning& t) {
             else if (key == "zcols" && val > 0) t.fft_zcols = val;
             else if (key == "xrows" && val > 0) t.fft_xrows = val;
             else if (key == "xy") t.fft_xy = val != 0;
+            else if (key == "xplan") t.fft_xplan = val != 0;
         }
         pos = end + 1;
     }

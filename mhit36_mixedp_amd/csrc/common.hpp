@@ -65,6 +65,10 @@ struct Tuning {
     int fft_xy = 0;       // x and y transforms of a z plane in one launch where
                           // compiled (n = 32 ... 128 FP32, 32 ... 96 FP64);
                           // mhit_create: 1 there
+    int fft_xplan = 0;    // radix plan of the separate x passes: 0 radix 8 first
+                          // (512 = 8.8.8), 1 radix 4 until 64 points remain
+                          // (512 = 4.4.4.8); the plans differ at n = 128,
+                          // 256, 384, 512
 };
 
 // Per-run constants passed by value to kernels.

@@ -178,7 +178,7 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
         if ((rc = launch_ns_march_batch(gp.data(), (int)gp.size(), ar, st))) return rc;
     auto prec_key = [](mhit_ctx* c) {
         return std::make_tuple(c->fp32, c->tune.fft_wave, fft_width(c, 0), fft_width(c, 1), fft_width(c, 2),
-                               c->tune.fft_xy);
+                               c->tune.fft_xy, c->tune.fft_xplan);
     };
     const auto by_prec = group_by(boxes, prec_key);
     for (auto& gp : by_prec)
@@ -528,6 +528,7 @@ int mhit_ensemble_autotune(mhit_ensemble* e, int steps, double* ms_per_step) {
             {&Tuning::fft_zcols, nullptr, widths(1)},
             {&Tuning::fft_xrows, nullptr, widths(2)},
             {&Tuning::fft_xy, nullptr, fft_xy_supported(n, c0->fp32) ? std::vector<int>{1, 0} : std::vector<int>{}},
+            {&Tuning::fft_xplan, nullptr, fft_xplan_differs(n) ? std::vector<int>{0, 1} : std::vector<int>{}},
         };
         for (const Param& p : params) {
             if (rc != MHIT_OK) break;

@@ -43,26 +43,42 @@ template <bool INV, typename V> __device__ __forceinline__ V c_rot(V a) {
     return r;
 }
 
-// Radix plan: factors of 3 first, then radix 8 while a radix-8 stage can
-// still end the plan, a 2 or a 4 in the middle, and 8 last whenever the
-// power-of-two part allows (512 = 8.8.8, 256 = 8.4.8, 128 = 8.2.8,
-// 96 = 3.4.8).  A radix-8 first or last stage with N/8 threads per column
-// takes its inputs from / leaves its outputs in registers in the order of
-// the global loads / stores (see k_fft_cols).
-constexpr int next_radix(int rem) {
+// Radix plan 0 (the default): factors of 3 first, then radix 8 while a
+// radix-8 stage can still end the plan, a 2 or a 4 in the middle, and 8
+// last whenever the power-of-two part allows (512 = 8.8.8, 256 = 8.4.8,
+// 128 = 8.2.8, 96 = 3.4.8).  A radix-8 first or last stage with N/8
+// threads per column takes its inputs from / leaves its outputs in
+// registers in the order of the global loads / stores (see k_fft_cols).
+// Plan 1 takes radix 4 until 64 points remain, then 8.8 (512 = 4.4.4.8,
+// 256 = 4.8.8, 128 = 4.4.8, 384 = 3.4.4.8): a stage more at 512 and 128,
+// with two radix-4 butterflies per thread instead of one radix-8
+// (Tuning::fft_xplan, the x passes only).
+constexpr int next_radix(int rem, int plan = 0) {
     return (rem % 3 == 0) ? 3
-         : (rem == 8 || rem % 64 == 0) ? 8
+         : (rem == 8 || (plan == 0 ? rem % 64 == 0 : rem == 64)) ? 8
+         : (plan == 1 && rem % 64 == 0) ? 4
          : (rem == 16) ? 2
          : (rem == 32) ? 4
          : (rem == 4) ? 4 : 2;
 }
-constexpr int last_radix(int n) {
+constexpr int last_radix(int n, int plan = 0) {
     int ns = 1, r = 1;
-    while (ns < n) { r = next_radix(n / ns); ns *= r; }
+    while (ns < n) { r = next_radix(n / ns, plan); ns *= r; }
     return r;
 }
-template <int REM> struct NextRadix {
-    static constexpr int value = next_radix(REM);
+// whether the two plans differ for n
+constexpr bool plans_differ(int n) {
+    int a = 1, b = 1;
+    while (a < n && b < n) {
+        const int ra = next_radix(n / a, 0), rb = next_radix(n / b, 1);
+        if (a != b || ra != rb) return true;
+        a *= ra;
+        b *= rb;
+    }
+    return a != b;
+}
+template <int REM, int PLAN = 0> struct NextRadix {
+    static constexpr int value = next_radix(REM, PLAN);
 };
 template <bool INV, typename V> __device__ __forceinline__ void dft2(V* a) {
     V t = a[0];
@@ -173,13 +189,14 @@ __device__ __forceinline__ void fft_stage(V* lds, int c, int t, const V* __restr
     __syncthreads();
 }
 
-// Stages from NS (points already combined) up to NS_END, each in LDS.
-template <typename V, int N, int NS, int C, int TPC, int LD, bool INV, int NS_END = N, int SW = 0>
+// Stages from NS (points already combined) up to NS_END, each in LDS, in
+// radix plan PLAN.
+template <typename V, int N, int NS, int C, int TPC, int LD, bool INV, int NS_END = N, int SW = 0, int PLAN = 0>
 __device__ __forceinline__ void fft_stages(V* lds, int c, int t, const V* __restrict__ tw) {
     if constexpr (NS < NS_END) {
-        constexpr int R = NextRadix<N / NS>::value;
+        constexpr int R = NextRadix<N / NS, PLAN>::value;
         fft_stage<V, N, R, NS, C, TPC, LD, INV, SW>(lds, c, t, tw);
-        fft_stages<V, N, NS * R, C, TPC, LD, INV, NS_END, SW>(lds, c, t, tw);
+        fft_stages<V, N, NS * R, C, TPC, LD, INV, NS_END, SW, PLAN>(lds, c, t, tw);
     }
 }
 

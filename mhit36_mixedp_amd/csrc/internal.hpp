@@ -212,6 +212,9 @@ bool fft_size_supported(int n);
 // poisson.hip: whether the x and y transforms of a z plane run as one launch
 // for this size and precision (Tuning::fft_xy)
 bool fft_xy_supported(int n, bool fp32);
+// poisson.hip: whether the x passes have a second radix plan for this size
+// (Tuning::fft_xplan: n = 128, 256, 384, 512)
+bool fft_xplan_differs(int n);
 // Launch widths compiled for an FFT pass (0: y passes, columns per
 // workgroup; 1: z pass, columns; 2: x passes, row pairs) of size n and
 // precision: out[0] is the default, the count is returned (at most 3).

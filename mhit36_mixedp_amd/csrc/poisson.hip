@@ -49,7 +49,7 @@ struct XFwdArgs {
     const cplx_t<T>* tw;
     const DevScalars* S;
 };
-template <typename T, int N, int IN, int P>
+template <typename T, int N, int IN, int P, int PL>
 __global__ __launch_bounds__((P * FftCfg<T, N>::TPC))
 void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
@@ -113,7 +113,7 @@ void k_fft_x_fwd(XFwdArgs<T> one, const XFwdArgs<T>* __restrict__ tab, int nbox)
         lds[lidx<LD, SW>(a, c)] = val;
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, false, N, SW>(lds, tid % P, tid / P, stw);
+    fft_stages<V, N, 1, P, TPC, LD, false, N, SW, PL>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (H * 2 * P + NT - 1) / NT; r++) {
@@ -152,7 +152,7 @@ struct XInvArgs {
     const cplx_t<T>* tw;
     const DevScalars* S;
 };
-template <typename T, int N, int P>
+template <typename T, int N, int P, int PL>
 __global__ __launch_bounds__((P * FftCfg<T, N>::TPC))
 void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox) {
     using V = cplx_t<T>;
@@ -198,7 +198,7 @@ void k_fft_x_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox)
         lds[lidx<LD, SW>(k0 ? 0 : N - k, c)] = zm;
     }
     __syncthreads();
-    fft_stages<V, N, 1, P, TPC, LD, true, N, SW>(lds, tid % P, tid / P, stw);
+    fft_stages<V, N, 1, P, TPC, LD, true, N, SW, PL>(lds, tid % P, tid / P, stw);
 
 #pragma unroll
     for (int r = 0; r < (N * 2 * P + NT - 1) / NT; r++) {
@@ -827,10 +827,21 @@ struct PoissonPasses {
         }, one, tab);
         if (rc) return rc;
         dim3 grid(N / (2 * P), N, nb), block(P * Cfg::TPC);
-        if (in == X_IN_READY) k_fft_x_fwd<T, N, X_IN_READY, P><<<grid, block, 0, st>>>(one, tab, nb);
-        else if (in == X_IN_DIV) k_fft_x_fwd<T, N, X_IN_DIV, P><<<grid, block, 0, st>>>(one, tab, nb);
-        else k_fft_x_fwd<T, N, X_IN_FIELD, P><<<grid, block, 0, st>>>(one, tab, nb);
+        if constexpr (plans_differ(N)) {
+            if (bx[0]->tune.fft_xplan) {
+                x_fwd_launch<P, 1>(grid, block, st, one, tab, nb, in);
+                return MHIT_OK;
+            }
+        }
+        x_fwd_launch<P, 0>(grid, block, st, one, tab, nb, in);
         return MHIT_OK;
+    }
+    template <int P, int PL>
+    static void x_fwd_launch(dim3 grid, dim3 block, hipStream_t st, const XFwdArgs<T>& one,
+                             const XFwdArgs<T>* tab, int nb, int in) {
+        if (in == X_IN_READY) k_fft_x_fwd<T, N, X_IN_READY, P, PL><<<grid, block, 0, st>>>(one, tab, nb);
+        else if (in == X_IN_DIV) k_fft_x_fwd<T, N, X_IN_DIV, P, PL><<<grid, block, 0, st>>>(one, tab, nb);
+        else k_fft_x_fwd<T, N, X_IN_FIELD, P, PL><<<grid, block, 0, st>>>(one, tab, nb);
     }
     // A column pass in the context's transform form (Tuning::fft_wave):
     // the wavefront form where it fits, else the register-direct / LDS
@@ -926,7 +937,13 @@ struct PoissonPasses {
         }, one, tab);
         if (rc) return rc;
         dim3 grid(N / (2 * P), N, nb), block(P * Cfg::TPC);
-        k_fft_x_inv<T, N, P><<<grid, block, 0, st>>>(one, tab, nb);
+        if constexpr (plans_differ(N)) {
+            if (bx[0]->tune.fft_xplan) {
+                k_fft_x_inv<T, N, P, 1><<<grid, block, 0, st>>>(one, tab, nb);
+                return MHIT_OK;
+            }
+        }
+        k_fft_x_inv<T, N, P, 0><<<grid, block, 0, st>>>(one, tab, nb);
         return MHIT_OK;
     }
     // x and y passes in one launch per z plane (Tuning::fft_xy) where the
@@ -1048,6 +1065,8 @@ bool fft_size_supported(int n) {
         default: return false;
     }
 }
+
+bool fft_xplan_differs(int n) { return plans_differ(n); }
 
 bool fft_xy_supported(int n, bool fp32) {
     switch (n) {

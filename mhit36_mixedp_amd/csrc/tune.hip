@@ -166,6 +166,7 @@ void to_public(const mhit_ctx* c, mhit_tuning* out, double ms) {
     out->fft_zcols = fft_width(c, 1);
     out->fft_xrows = fft_width(c, 2);
     out->fft_xy = (t.fft_xy && fft_xy_supported(c->n, c->fp32)) ? 1 : 0;
+    out->fft_xplan = (t.fft_xplan && fft_xplan_differs(c->n)) ? 1 : 0;
     out->ms_per_step = ms;
 }
 
@@ -184,7 +185,7 @@ bool valid(const mhit_ctx* c, const mhit_tuning* t) {
     return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
            rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1) &&
            valid_width(c, 0, t->fft_ycols) && valid_width(c, 1, t->fft_zcols) && valid_width(c, 2, t->fft_xrows) &&
-           (t->fft_xy == 0 || t->fft_xy == 1);
+           (t->fft_xy == 0 || t->fft_xy == 1) && (t->fft_xplan == 0 || t->fft_xplan == 1);
 }
 
 }  // namespace
@@ -230,6 +231,7 @@ int mhit_set_tuning(mhit_ctx* c, const mhit_tuning* t) {
     c->tune.fft_zcols = t->fft_zcols;
     c->tune.fft_xrows = t->fft_xrows;
     c->tune.fft_xy = t->fft_xy;
+    c->tune.fft_xplan = t->fft_xplan;
     clear_graphs(c);
     return MHIT_OK;
 }
@@ -268,6 +270,7 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
         {&c->tune.fft_zcols, widths(1), widths(1).size() > 1},
         {&c->tune.fft_xrows, widths(2), widths(2).size() > 1},
         {&c->tune.fft_xy, {1, 0}, fft_xy_supported(n, c->fp32)},
+        {&c->tune.fft_xplan, {0, 1}, fft_xplan_differs(n)},
     };
     double best_ms = 0.0;
     rc = MHIT_OK;

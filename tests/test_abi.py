@@ -65,7 +65,7 @@ int main(void) {
   printf("mhit_diag %zu\n", sizeof(mhit_diag));
   printf("mhit_profile %zu\n", sizeof(mhit_profile));
   printf("mhit_tuning %zu\n", sizeof(mhit_tuning));
-  F(mhit_tuning, fft_wave) F(mhit_tuning, fft_xrows) F(mhit_tuning, fft_xy) F(mhit_tuning, ms_per_step)
+  F(mhit_tuning, fft_wave) F(mhit_tuning, fft_xrows) F(mhit_tuning, fft_xy) F(mhit_tuning, fft_xplan) F(mhit_tuning, ms_per_step)
   F(mhit_params, dt) F(mhit_params, npart) F(mhit_params, nx) F(mhit_params, cfl_abort)
   F(mhit_diag, ke) F(mhit_diag, nan_count) F(mhit_profile, ms) F(mhit_profile, bytes_per_call)
   return 0;

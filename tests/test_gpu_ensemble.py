@@ -189,7 +189,7 @@ def test_ensemble_autotune_restores_state_and_matches_solo():
             print(i, t, flush=True)
             s.step(2)
             s.set_tuning(fft_wave=t["fft_wave"], fft_ycols=t["fft_ycols"], fft_zcols=t["fft_zcols"],
-                         fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"])
+                         fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"], fft_xplan=t["fft_xplan"])
         ens.step(3)
         for s in solos:
             s.step(3)

@@ -221,3 +221,30 @@ def test_xy_in_one_launch_and_apart_match_numpy(n, prec):
             out[v] = (F, y, p)
     for a, b in zip(out[1], out[0]):
         assert relerr(a, b) < tol
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("n", [128, 256])
+def test_x_radix_plans_match_numpy(n, prec):
+    """Both radix plans of the x passes (fft_xplan 0: 8.2.8 / 8.4.8, 1:
+    4.4.8 / 4.8.8), with the x transforms in launches of their own, against
+    numpy: forward, inverse and the Poisson solve."""
+    print(f"[fft xplan] n={n} prec={prec}", flush=True)
+    rng = np.random.default_rng(950 + n + prec)
+    x = rng.standard_normal((n, n, n))
+    tol = 1e-12 if prec == 0 else 2e-6
+    X = np.fft.rfftn(x)
+    k = np.fft.fftfreq(n, 1.0 / n)
+    kz, ky, kx = np.meshgrid(k, k, k[: n // 2 + 1], indexing="ij")
+    k2 = kx ** 2 + ky ** 2 + kz ** 2
+    k2[0, 0, 0] = 1.0
+    ph = -X / k2
+    ph[0, 0, 0] = 0.0
+    ref = np.fft.irfftn(ph, s=(n, n, n), axes=(0, 1, 2))
+    with _solver(n, prec) as s:
+        for plan in (1, 0):
+            s.set_tuning(fft_xy=0, fft_xplan=plan)
+            assert s.tuning()["fft_xplan"] == plan
+            assert relerr(s.fft_r2c(x), X) < tol, plan
+            assert relerr(s.fft_c2r(X), x * n ** 3) < tol, plan
+            assert relerr(s.poisson_solve(x), ref) < (1e-12 if prec == 0 else 1e-5), plan

@@ -28,7 +28,7 @@ def test_autotune_restores_state_and_results_are_shape_independent():
         # are the FFT widths, compiled with alternatives from 64 up); the
         # launch shapes change no bit
         b.set_tuning(fft_wave=t["fft_wave"], fft_ycols=t["fft_ycols"], fft_zcols=t["fft_zcols"],
-                     fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"])
+                     fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"], fft_xplan=t["fft_xplan"])
         assert t["ms_per_step"] > 0 and t["fft_xy"] in (0, 1)
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
         assert t["ns_my"] in (8, 16, 32) and t["ac_my"] in (8, 16, 32) and t["fft_wave"] in (0, 1)
@@ -162,3 +162,17 @@ def test_fft_xy_default_and_validation(n, prec, compiled):
         assert s.tuning()["fft_xy"] == int(compiled)
         with pytest.raises(MhitError):
             s.set_tuning(fft_xy=2)
+
+
+@pytest.mark.parametrize("n,two", [(64, False), (96, False), (128, True), (256, True), (192, False)])
+def test_fft_xplan_validation(n, two):
+    """The x passes have a second radix plan only where it differs from the
+    default (n = 128, 256, 384, 512); elsewhere the context reports 0."""
+    from mhit36_mixedp_amd import MhitError, Solver
+    print(f"[tuning] fft_xplan n={n}", flush=True)
+    with Solver(nx=n, poisson_precision=1) as s:
+        assert s.tuning()["fft_xplan"] == 0
+        s.set_tuning(fft_xplan=1)
+        assert s.tuning()["fft_xplan"] == int(two)
+        with pytest.raises(MhitError):
+            s.set_tuning(fft_xplan=-1)
