@@ -294,6 +294,9 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the ensemble workload")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched ensemble and print its JSON (for kernel traces)")
+    ap.add_argument("--one-size-only", action="store_true",
+                    help="run only the one-size ensembles (16 boxes each of 64^3, 96^3, 128^3) "
+                         "and print their box-steps/s (for kernel traces)")
     ap.add_argument("--autotune", action="store_true",
                     help="let mhit_autotune pick the launch shapes before timing (default: the "
                          "library's size-based defaults, which the kernel trace under profiles/ uses)")
@@ -309,6 +312,10 @@ def main():
     if args.ensemble_only:
         log(f"[bench] {version()}  batched ensemble only")
         print(json.dumps({"ensemble": ensemble_leg(args, batched_only=True)}), flush=True)
+        return
+    if args.one_size_only:
+        log(f"[bench] {version()}  one-size ensembles only")
+        print(json.dumps({"one_size_box_steps_per_s": one_size_rates(args)}), flush=True)
         return
     log(f"[bench] {version()}  n={args.n} drops={args.ndrops} tracers={args.ntracers}")
 
