@@ -267,7 +267,7 @@ int launch_ns_march_batch(mhit_ctx* const* boxes, int nb, ArgArena& arena, hipSt
 
 // tracers.hip
 int launch_tracers(mhit_ctx* c, hipStream_t st);
-int sort_tracers(mhit_ctx* c);          // stable cell-order (Morton) permutation
+int sort_tracers(mhit_ctx* c);          // stable cell-order permutation
 void free_sort_buffers(mhit_ctx* c);
 
 // init.hip

@@ -1,29 +1,23 @@
-"""The tracer re-ordering: a stable sort of the tracers by the Morton code
-of their cell (a hand-written LSD radix sort, 8 bits per pass).
+"""The tracer re-ordering: a stable sort of the tracers by the index of
+their cell, (k n + j) n + i (a hand-written LSD radix sort, 8 bits per
+pass).
 
 The step after an upload re-orders first.  From rest the first (Euler)
 step moves no tracer, so after one step the tracer columns must be exactly
 the uploaded ones permuted by numpy's stable argsort of the cell codes:
 equal codes keep their upload order.  The sizes cover a partial last tile
 (tiles are 4096 tracers), a single tile, two, three and four digit passes
-(the code has 3 log2(n) bits), and many tracers per cell."""
+(the passes cover 3 ceil(log2 n) bits), and many tracers per cell."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
 
-def _spread3(v):
-    v = v.astype(np.uint64) & np.uint64(0x3FF)
-    for s, m in ((16, 0x030000FF), (8, 0x0300F00F), (4, 0x030C30C3), (2, 0x09249249)):
-        v = (v | (v << np.uint64(s))) & np.uint64(m)
-    return v
-
-
 def _cell_codes(t, n):
     dx = 2.0 * np.pi / n
     c = [np.mod(np.floor(t[a] / dx).astype(np.int64), n) for a in ("x", "y", "z")]
-    return _spread3(c[0]) | (_spread3(c[1]) << np.uint64(1)) | (_spread3(c[2]) << np.uint64(2))
+    return (c[2] * n + c[1]) * n + c[0]
 
 
 @pytest.mark.parametrize("n,ntr", [(8, 3000), (16, 4096), (64, 1_000_003), (256, 262_145), (512, 4_194_304)])
@@ -47,4 +41,4 @@ def test_reorder_is_stable_cell_order(n, ntr):
     order = np.argsort(_cell_codes(t0, n), kind="stable")
     for col in ("tag", "x", "y", "z"):
         assert np.array_equal(t1[col], t0[col][order]), col
-    assert np.all(np.diff(_cell_codes(t1, n).astype(np.int64)) >= 0)
+    assert np.all(np.diff(_cell_codes(t1, n)) >= 0)
