@@ -242,9 +242,11 @@ typedef struct mhit_tuning {
 int mhit_get_tuning(mhit_ctx* ctx, mhit_tuning* out);
 int mhit_set_tuning(mhit_ctx* ctx, const mhit_tuning* t);
 /* Time candidate shapes and transform forms on the context's own state
- * (`steps` timed steps per candidate), keep the fastest, and restore the
- * state: the run then continues exactly as if the call had not been made
- * (apart from the rounding of the transform form it picks).               */
+ * (`steps` timed steps per candidate), one setting at a time, keeping the
+ * fastest; then time the chosen settings and the starting ones
+ * alternately, twice each, and keep the faster set.  The state is
+ * restored: the run then continues exactly as if the call had not been
+ * made (apart from the rounding of the transform forms it picks).         */
 int mhit_autotune(mhit_ctx* ctx, int steps, mhit_tuning* out);
 
 /* Drop statistics from the phase field (post-processing of the reference,
@@ -348,15 +350,17 @@ int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);
 int mhit_ensemble_step(mhit_ensemble* ens, int nsteps);
 int mhit_ensemble_set_batched(mhit_ensemble* ens, int32_t on);
 /* Batched form: tune the launch shapes per grid size and Poisson precision
- * (z chunk of the fused passes, their tile rows, the FFT transform form and
- * launch widths; the same value for every box of the class), one setting
- * at a time, each candidate timed over `steps` ensemble steps on the live
- * state; keeps the fastest and restores every box's state (as
- * mhit_autotune does for one box; the reference's analogue is cuDecomp's
- * start-up autotune, multi/main.f90:80-136).  *ms_per_step returns the
- * ensemble step time with the chosen settings, which each box's
- * mhit_get_tuning reports.  The transform form changes rounding only;
- * mhit_ensemble_set_batched resets the shapes.                            */
+ * (z chunk of the fused passes, their tile rows, the FFT transform forms,
+ * the x passes' radix plan and the launch widths; the same value for every
+ * box of the class), one setting at a time, each candidate timed over
+ * `steps` ensemble steps on the live state; then the chosen settings and
+ * the starting ones are timed alternately, twice each, and the faster set
+ * is kept.  Every box's state is restored (as mhit_autotune does for one
+ * box; the reference's analogue is cuDecomp's start-up autotune,
+ * multi/main.f90:80-136).  *ms_per_step returns the ensemble step time
+ * with the kept settings, which each box's mhit_get_tuning reports.  The
+ * transform forms change rounding only; mhit_ensemble_set_batched resets
+ * the shapes.                                                             */
 int mhit_ensemble_autotune(mhit_ensemble* ens, int steps, double* ms_per_step);
 int mhit_ensemble_destroy(mhit_ensemble* ens);
 
