@@ -383,7 +383,7 @@ void k_fft_yx_inv(XInvArgs<T> one, const XInvArgs<T>* __restrict__ tab, int nbox
     fft_stages<V, N, 1, H, TPC, H, true>(sy, tid % H, tid / H, stw);
     // x C2R input of row pair c (as k_fft_x_inv): Z[k] = G[2c][k] + i G[2c+1][k]
     // over the Hermitian extension; G[y][0] = Re sy[y][0], G[y][H] = Im sy[y][0]
-    V zh;
+    V zh{};
     if (tid < P) {
         const V a0 = sy[(2 * tid) * H], b0 = sy[(2 * tid + 1) * H];
         zh.x = a0.y;
