@@ -350,12 +350,12 @@ int mhit_ensemble_box(mhit_ensemble* ens, int32_t i, mhit_ctx** box);
 int mhit_ensemble_step(mhit_ensemble* ens, int nsteps);
 int mhit_ensemble_set_batched(mhit_ensemble* ens, int32_t on);
 /* Batched form: tune the launch shapes per grid size and Poisson precision
- * (z chunk of the fused passes, their tile rows, the FFT transform forms,
- * the x passes' radix plan and the launch widths; the same value for every
- * box of the class), one setting at a time, each candidate timed over
- * `steps` ensemble steps on the live state; then the chosen settings and
- * the starting ones are timed alternately, twice each, and the faster set
- * is kept.  Every box's state is restored (as mhit_autotune does for one
+ * (z chunk of the fused passes, their tile rows and XCD order, the FFT
+ * transform forms, the x passes' radix plan and the launch widths; the
+ * same value for every box of the class), one setting at a time, each
+ * candidate timed over `steps` ensemble steps on the live state; then the
+ * chosen settings and the starting ones are timed alternately, twice
+ * each, and the faster set is kept.  Every box's state is restored (as mhit_autotune does for one
  * box; the reference's analogue is cuDecomp's start-up autotune,
  * multi/main.f90:80-136).  *ms_per_step returns the ensemble step time
  * with the kept settings, which each box's mhit_get_tuning reports.  The

@@ -48,7 +48,8 @@ struct DevScalars {
 struct Tuning {
     int ns_mz = 64;       // planes per workgroup, NS pass (set by grid size at create)
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
-    int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3)
+    int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3;
+                          // 1 for boxes batched in an ensemble)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
     int ns_my = 16;       // tile rows, NS pass: 16 or 8 (64 columns) or 32 (32
                           // columns); mhit_create: 8 for n <= 64, 32 where n

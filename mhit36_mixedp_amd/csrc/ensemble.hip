@@ -189,7 +189,7 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
 }
 
 // Launch shapes for boxes batched in an ensemble: 64 x 16 tiles (32 x 32
-// where n is not a multiple of 64) and z
+// where n is not a multiple of 64) in XCD order and z
 // chunks of min(n, 128) planes, then chunks halved size class by size
 // class (the class with the longest chunks first, ties to the larger grid;
 // never below 8 planes) until the NS launches of one step hold at least
@@ -525,6 +525,7 @@ int mhit_ensemble_autotune(mhit_ensemble* e, int steps, double* ms_per_step) {
             {&Tuning::ns_mz, &Tuning::ac_mz, zc},
             {&Tuning::ns_my, nullptr, {16, 8, 32}},
             {&Tuning::ac_my, nullptr, {16, 8, 32}},
+            {&Tuning::xcd, nullptr, {0, 1}},
             {&Tuning::fft_wave, nullptr, {1, 0}},
             {&Tuning::fft_ycols, nullptr, widths(0)},
             {&Tuning::fft_zcols, nullptr, widths(1)},
