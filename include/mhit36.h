@@ -194,7 +194,8 @@ int mhit_set_ab2_state(mhit_ctx* ctx, double alpha, double beta, double gumax);
 typedef struct mhit_tuning {
     int32_t ns_mz;        /* z planes per workgroup, fused NS pass          */
     int32_t ac_mz;        /* z planes per workgroup, fused ACDI pass        */
-    int32_t xcd;          /* 1: XCD-aware workgroup order, fused passes     */
+    int32_t xcd;          /* 1: XCD-aware workgroup order, fused passes (the
+                             default up to 256^3 and in ensembles)          */
     int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
     int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8 (64
                              columns) or 32 (32 columns)                    */

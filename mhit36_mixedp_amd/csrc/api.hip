@@ -132,10 +132,12 @@ void default_tuning(mhit_ctx* c, bool batched, bool env) {
     c->tune.ac_mz = zc;
     c->tune.ns_my = rows;
     c->tune.ac_my = rows;
-    // XCD tile order in batched launches: neighbouring tiles of a box then
-    // share an XCD's L2 (the 64^3 / 96^3 / 128^3 ensemble 18.78-18.84k ->
-    // 18.87-18.96k box-steps/s); a box alone measured neutral and keeps 0
-    c->tune.xcd = batched ? 1 : 0;
+    // XCD tile order: neighbouring tiles of a box then share an XCD's L2.
+    // Batched (the 64^3 / 96^3 / 128^3 ensemble 18.78-18.84k -> 18.87-
+    // 18.96k box-steps/s) and a box alone up to 256^3 (ms per step: 64^3
+    // 0.049 -> 0.046, 128^3 multiphase FP64 0.190 -> 0.183, 256^3 0.764 ->
+    // 0.751); at 512^3 it measured slower (7.56 -> 7.68 ms) and stays off
+    c->tune.xcd = (batched || n <= 256) ? 1 : 0;
     // the wavefront form of the column transforms pays in FP32 (the z pass);
     // with the FP64 Poisson the register-direct form measured faster
     // (512^3: 8.84-8.89 -> 8.77 ms per step)

@@ -48,8 +48,8 @@ struct DevScalars {
 struct Tuning {
     int ns_mz = 64;       // planes per workgroup, NS pass (set by grid size at create)
     int ac_mz = 64;       // planes per workgroup, ACDI pass (likewise)
-    int xcd = 0;          // XCD-aware tile order (measured neutral at 512^3;
-                          // 1 for boxes batched in an ensemble)
+    int xcd = 0;          // XCD-aware tile order (mhit_create: 1 up to 256^3
+                          // and for boxes batched in an ensemble, 0 above)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
     int ns_my = 16;       // tile rows, NS pass: 16 or 8 (64 columns) or 32 (32
                           // columns); mhit_create: 8 for n <= 64, 32 where n
