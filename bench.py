@@ -380,6 +380,12 @@ def main():
         avg = r["ms"] / r["calls"]
         gbps = r["bytes_per_call"] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
         log(f"   {r['stage']:<14} {avg:8.3f} ms  {r['bytes_per_call'] / 1e9:7.3f} GB  {gbps:8.1f} GB/s")
+    if prof64:
+        log(f"[bench] per-stage (FP64 Poisson, {args.profile_steps} profiled steps):")
+        for r in prof64:
+            avg = r["ms"] / r["calls"]
+            gbps = r["bytes_per_call"] / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+            log(f"   {r['stage']:<14} {avg:8.3f} ms  {r['bytes_per_call'] / 1e9:7.3f} GB  {gbps:8.1f} GB/s")
     dom = max(prof32, key=lambda r: r["ms"])
     dom_avg_ms = dom["ms"] / dom["calls"]
     dom_gbps = dom["bytes_per_call"] / (dom_avg_ms * 1e-3) / 1e9
@@ -441,6 +447,8 @@ def main():
             "stream_mix": "pointwise kernel with the NS pass's 7 FP64 read + 6 FP64 write streams",
         },
         "stages": {r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof32},
+        "fp64_poisson_stages": ({r["stage"]: round(r["ms"] / r["calls"], 4) for r in prof64}
+                                if prof64 else None),
         "tuning": s_tuning,
         "ensemble": ens,
         "small_box": small,
