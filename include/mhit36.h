@@ -198,7 +198,7 @@ typedef struct mhit_tuning {
                              default up to 256^3 and in ensembles)          */
     int32_t sort_every;   /* tracer re-ordering period in steps (0: never)  */
     int32_t ns_my;        /* tile rows of the fused NS pass: 16 or 8 (64
-                             columns) or 32 (32 columns)                    */
+                             columns), 32 (32 columns) or 10 (96 columns)   */
     int32_t ac_my;        /* tile rows of the fused ACDI pass: the same     */
     int32_t fft_wave;     /* 1: column transforms of eight columns (the z
                              pass; the y pass in FP64) in the wavefront form,

@@ -51,9 +51,10 @@ struct Tuning {
     int xcd = 0;          // XCD-aware tile order (mhit_create: 1 up to 256^3
                           // and for boxes batched in an ensemble, 0 above)
     int sort_every = 10;  // steps between tracer re-orderings (0: never)
-    int ns_my = 16;       // tile rows, NS pass: 16 or 8 (64 columns) or 32 (32
-                          // columns); mhit_create: 8 for n <= 64, 32 where n
-                          // is not a multiple of 64 (96)
+    int ns_my = 16;       // tile rows, NS pass: 16 or 8 (64 columns), 32 (32
+                          // columns) or 10 (96 columns); mhit_create: 8 for
+                          // n <= 64, 10 where n is a multiple of 96 but not
+                          // of 64 (96)
     int ac_my = 16;       // tile rows, ACDI pass (the same choices)
     int fft_wave = 1;     // column transforms of eight columns in the wavefront
                           // form (lane/register transposes by permlane and DPP)
@@ -82,8 +83,9 @@ struct Grid {
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 
-// Columns of a fused pass's tile for its Tuning rows (64 x 16, 64 x 8, 32 x 32).
-inline int tile_cols(int rows) { return rows == 32 ? 32 : 64; }
+// Columns of a fused pass's tile for its Tuning rows (64 x 16, 64 x 8,
+// 32 x 32, 96 x 10).
+inline int tile_cols(int rows) { return rows == 32 ? 32 : (rows == 10 ? 96 : 64); }
 
 // periodic neighbours
 __device__ __forceinline__ int wrap_p(int i, int n) { return i + 1 == n ? 0 : i + 1; }

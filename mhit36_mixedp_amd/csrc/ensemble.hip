@@ -188,7 +188,7 @@ static int enqueue_size_class(const std::vector<mhit_ctx*>& boxes, ArgArena& ar,
     return launch_finalize_batch(boxes.data(), (int)boxes.size(), ar, st);
 }
 
-// Launch shapes for boxes batched in an ensemble: 64 x 16 tiles (32 x 32
+// Launch shapes for boxes batched in an ensemble: 64 x 16 tiles (96 x 10
 // where n is not a multiple of 64) in XCD order and z
 // chunks of min(n, 128) planes, then chunks halved size class by size
 // class (the class with the longest chunks first, ties to the larger grid;
@@ -523,8 +523,8 @@ int mhit_ensemble_autotune(mhit_ensemble* e, int steps, double* ms_per_step) {
         };
         const std::vector<Param> params = {
             {&Tuning::ns_mz, &Tuning::ac_mz, zc},
-            {&Tuning::ns_my, nullptr, {16, 8, 32}},
-            {&Tuning::ac_my, nullptr, {16, 8, 32}},
+            {&Tuning::ns_my, nullptr, {16, 8, 32, 10}},
+            {&Tuning::ac_my, nullptr, {16, 8, 32, 10}},
             {&Tuning::xcd, nullptr, {0, 1}},
             {&Tuning::fft_wave, nullptr, {1, 0}},
             {&Tuning::fft_ycols, nullptr, widths(0)},

@@ -48,8 +48,7 @@ __device__ __forceinline__ int ring4(int k) { return ((k % 4) + 4) % 4; }
 // pass reads its neighbours' previous rhs, so the new AB2 history goes to
 // the other buffer of a ping-pong pair.
 // Tile: TMX columns in x by TMY rows in y (Tuning::ns_my: 64 x 16, 64 x 8,
-// or 32 x 32 for grids that are not a multiple of 64), one thread per
-// column; z chunk: Tuning::ns_mz.  The velocity planes carry x, y in
+// 32 x 32 or 96 x 10), one thread per column; z chunk: Tuning::ns_mz.  The velocity planes carry x, y in
 // [-1, M + 1], phi a two-point halo.
 
 // Arguments of one box.  A launch covers one box (`one`) or, for an
@@ -339,7 +338,7 @@ void k_ns_march(NsArgs<TD> one, const NsArgs<TD>* __restrict__ tab, int nbox) {
 // =====================================================================
 // ACDI
 // =====================================================================
-// Tile: TMX columns by TMY rows (Tuning::ac_my: 64 x 16, 64 x 8 or
+// Tile: TMX columns by TMY rows (Tuning::ac_my: 64 x 16, 64 x 8, 96 x 10 or
 // 32 x 32); z chunk: Tuning::ac_mz.  phi ring with a one-point halo, psi
 // ring with two.
 
@@ -604,6 +603,7 @@ static void go_ns(mhit_ctx* c, const NsArgs<TD>& one, const NsArgs<TD>* tab, int
     };
     if (c->tune.ns_my == 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 64>{});
     else if (c->tune.ns_my == 32) go(std::integral_constant<int, 32>{}, std::integral_constant<int, 32>{});
+    else if (c->tune.ns_my == 10) go(std::integral_constant<int, 10>{}, std::integral_constant<int, 96>{});
     else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 64>{});
 }
 
@@ -659,6 +659,7 @@ static void go_acdi(mhit_ctx* c, const AcdiArgs& one, const AcdiArgs* tab, int n
     };
     if (c->tune.ac_my == 8) go(std::integral_constant<int, 8>{}, std::integral_constant<int, 64>{});
     else if (c->tune.ac_my == 32) go(std::integral_constant<int, 32>{}, std::integral_constant<int, 32>{});
+    else if (c->tune.ac_my == 10) go(std::integral_constant<int, 10>{}, std::integral_constant<int, 96>{});
     else go(std::integral_constant<int, 16>{}, std::integral_constant<int, 64>{});
 }
 

@@ -181,7 +181,7 @@ bool valid_width(const mhit_ctx* c, int pass, int w) {
 }
 
 bool valid(const mhit_ctx* c, const mhit_tuning* t) {
-    auto rows = [](int r) { return r == 8 || r == 16 || r == 32; };
+    auto rows = [](int r) { return r == 8 || r == 16 || r == 32 || r == 10; };
     return t->ns_mz > 0 && t->ac_mz > 0 && (t->xcd == 0 || t->xcd == 1) && t->sort_every >= 0 &&
            rows(t->ns_my) && rows(t->ac_my) && (t->fft_wave == 0 || t->fft_wave == 1) &&
            valid_width(c, 0, t->fft_ycols) && valid_width(c, 1, t->fft_zcols) && valid_width(c, 2, t->fft_xrows) &&
@@ -260,9 +260,9 @@ int mhit_autotune(mhit_ctx* c, int steps, mhit_tuning* out) {
     };
     struct Param { int* field; std::vector<int> cand; bool active; };
     std::vector<Param> params = {
-        {&c->tune.ns_my, {16, 8, 32}, true},
+        {&c->tune.ns_my, {16, 8, 32, 10}, true},
         {&c->tune.ns_mz, zlist({4, 8, 16, 32, 64, 128}), true},
-        {&c->tune.ac_my, {16, 8, 32}, c->P.phiflag != 0},
+        {&c->tune.ac_my, {16, 8, 32, 10}, c->P.phiflag != 0},
         {&c->tune.ac_mz, zlist({4, 8, 16, 32, 64, 128}), c->P.phiflag != 0},
         {&c->tune.xcd, {0, 1}, true},
         {&c->tune.fft_wave, {1, 0}, true},

@@ -31,7 +31,7 @@ def test_autotune_restores_state_and_results_are_shape_independent():
                      fft_xrows=t["fft_xrows"], fft_xy=t["fft_xy"], fft_xplan=t["fft_xplan"])
         assert t["ms_per_step"] > 0 and t["fft_xy"] in (0, 1)
         assert t["ns_mz"] in (4, 8, 16, 32) and t["ac_mz"] in (4, 8, 16, 32) and t["xcd"] in (0, 1)
-        assert t["ns_my"] in (8, 16, 32) and t["ac_my"] in (8, 16, 32) and t["fft_wave"] in (0, 1)
+        assert t["ns_my"] in (8, 16, 32, 10) and t["ac_my"] in (8, 16, 32, 10) and t["fft_wave"] in (0, 1)
         assert a.tuning()["ns_mz"] == t["ns_mz"]
         # every field, the last projection's pressure (4) included
         for f in range(1, 10):
@@ -59,10 +59,16 @@ def test_autotune_restores_state_and_results_are_shape_independent():
     # (with the x and y transforms in separate launches, on both solvers)
     (64, dict(fft_xrows=4, fft_xy=0)),
     (96, dict(fft_xrows=16, fft_xy=0)),
-    # 32 x 32 tiles (the default at 96^3), full and partial (48 = 32 + 16)
+    # 32 x 32 tiles, full and partial (48 = 32 + 16)
     (64, dict(ns_my=32, ac_my=32, ns_mz=16, ac_mz=16)),
     (96, dict(ns_my=16, ac_my=8)),
     (48, dict(ns_my=32, ac_my=32, ns_mz=8, ac_mz=4, xcd=1)),
+    # 96 x 10 tiles (the default at 96^3): a partial last row of tiles
+    # (96 = 9 x 10 + 6), and partial in x and y at 64 and 48
+    (96, dict(ns_my=32, ac_my=32)),
+    (96, dict(ns_my=10, ac_my=16, ns_mz=12, ac_mz=24, xcd=0)),
+    (64, dict(ns_my=10, ac_my=10, ns_mz=16, ac_mz=8)),
+    (48, dict(ns_my=10, ac_my=10, ns_mz=8, ac_mz=4, xcd=1)),
 ])
 def test_set_tuning_is_bitwise_neutral(n, shape):
     from mhit36_mixedp_amd import Solver
